@@ -1,0 +1,759 @@
+// C ABI of libemc: context lifetime, host<->device transfers and the stage sequence of one EMC
+// iteration (main.cpp:239-480) on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "emc_internal.h"
+#include "host/host_internal.h"
+
+const char *emc_host_last_error();
+
+namespace emc {
+int64_t softmax_part_len(int S, int I);
+}
+
+using namespace emc;
+
+namespace {
+
+emc_status fail(emc_ctx *c, emc_status st, const std::string &msg)
+{
+    if (c) c->err = msg;
+    else emc_host_fail(st, msg.c_str());
+    return st;
+}
+
+#define HIP_TRY(c, expr)                                                                            \
+    do {                                                                                            \
+        hipError_t e_ = (expr);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return fail((c), e_ == hipErrorOutOfMemory ? EMC_E_OOM : EMC_E_HIP,                     \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+    } while (0)
+
+#define EMC_TRY(expr)                                                                               \
+    do {                                                                                            \
+        emc_status s_ = (expr);                                                                     \
+        if (s_ != EMC_OK) return s_;                                                                \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, int64_t n)
+{
+    *p = nullptr;
+    if (n <= 0) return hipSuccess;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+    if (e != hipSuccess) return e;
+    return hipMemset(*p, 0, sizeof(T) * (size_t)n);
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+// Blur taps g[d] = (1/L) sum_k exp(-2 pi^2 sigma^2 min(k,L-k)^2 / L^2) cos(2 pi k d / L).
+std::vector<float> blur_taps(int L, double sigma)
+{
+    std::vector<float> g(L);
+    for (int d = 0; d < L; ++d) {
+        double acc = 0.0;
+        for (int k = 0; k < L; ++k) {
+            const int kk = std::min(k, L - k);
+            acc += std::exp(-2.0 * M_PI * M_PI * sigma * sigma * (double)kk * kk / ((double)L * L)) *
+                   std::cos(2.0 * M_PI * (double)k * d / L);
+        }
+        g[d] = (float)(acc / L);
+    }
+    return g;
+}
+
+void ev_start(emc_ctx *c, int stage)
+{
+    if (c->opt.timing) (void)hipEventRecord(c->ev[2 * stage], c->stream);
+}
+void ev_stop(emc_ctx *c, int stage)
+{
+    if (c->opt.timing) (void)hipEventRecord(c->ev[2 * stage + 1], c->stream);
+}
+
+emc_status sync_scalars(emc_ctx *c)
+{
+    HIP_TRY(c, hipMemcpyAsync(c->scal_h, c->scal, sizeof(double) * SC_N, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return EMC_OK;
+}
+
+int n_chunks(const emc_ctx *c) { return (c->S + c->chunk - 1) / c->chunk; }
+
+emc_status rebuild_active(emc_ctx *c, const float *coords_h, const int32_t *mask_h)
+{
+    std::vector<int32_t> pix;
+    for (int p = 0; p < c->P; ++p)
+        if (mask_h[p] != 0) pix.push_back(p);
+    const int Pa = (int)pix.size();
+    if (Pa == 0) return fail(c, EMC_E_INVALID, "mask has no active pixel");
+    const int Pp = (int)round_up(Pa, kPadP);
+    c->Pa = Pa;
+    c->Pp = Pp;
+    std::vector<float> axyz((size_t)3 * Pp, 0.0f);
+    std::vector<int32_t> pixp(Pp, -1);
+    for (int j = 0; j < Pa; ++j) {
+        pixp[j] = pix[j];
+        if (coords_h)
+            for (int a = 0; a < 3; ++a) axyz[(size_t)a * Pp + j] = coords_h[(size_t)a * c->P + pix[j]];
+    }
+    HIP_TRY(c, hipMemcpy(c->pix, pixp.data(), sizeof(int32_t) * Pp, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->axyz, axyz.data(), sizeof(float) * 3 * Pp, hipMemcpyHostToDevice));
+    return EMC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+
+extern "C" void emc_default_options(emc_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->diff = EMC_DIFF_TRUE_POISSON;
+    o->interp = EMC_INTERP_TRILINEAR;
+    o->precision = EMC_PREC_BF16X2;
+    o->insert = EMC_INSERT_FUSED;
+    o->sigma_start = 0.4;
+    o->sigma_final = 0.12;
+    o->sigma_half_life = 25;
+    o->device = 0;
+    o->blur_sigma = 0.0;
+    o->fluence = 0.0;
+    o->timing = 0;
+}
+
+extern "C" const char *emc_last_error(const emc_ctx *c)
+{
+    if (c) return c->err.c_str();
+    return emc_host_last_error();
+}
+
+extern "C" emc_status emc_device_info(int32_t device, char *name, int32_t name_len, int32_t *n_cu,
+                                      int64_t *hbm_bytes)
+{
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return fail(nullptr, EMC_E_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (int64_t)prop.totalGlobalMem;
+    return EMC_OK;
+}
+
+extern "C" void emc_destroy(emc_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipSetDevice(c->opt.device);
+    dfree(c->q); dfree(c->rotm); dfree(c->logw); dfree(c->coords); dfree(c->mask); dfree(c->axyz); dfree(c->pix);
+    dfree(c->K32); dfree(c->KT32); dfree(c->K16); dfree(c->K16lo); dfree(c->KT16); dfree(c->KT16lo); dfree(c->sumK);
+    dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
+    dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
+    dfree(c->A0); dfree(c->A1); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
+    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
+    if (c->scal_h) (void)hipHostFree(c->scal_h);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ctx **out)
+{
+    if (!d || !out) return fail(nullptr, EMC_E_INVALID, "emc_create: null argument");
+    *out = nullptr;
+    emc_options opt;
+    if (o) opt = *o;
+    else emc_default_options(&opt);
+    if (d->model_side < 4 || d->model_side > 1024 || d->n_rotations < 1 || d->n_patterns < 1 || d->slice_chunk < 0)
+        return fail(nullptr, EMC_E_INVALID, "emc_create: bad dimensions");
+    if (opt.diff < 0 || opt.diff > 3 || opt.interp < 0 || opt.interp > 1 || opt.precision < 0 || opt.precision > 2 ||
+        opt.insert < 0 || opt.insert > 1)
+        return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
+    if (opt.precision != EMC_PREC_FP32 && opt.diff != EMC_DIFF_TRUE_POISSON)
+        return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: MFMA precision modes need diff = TRUE_POISSON");
+    if (opt.sigma_half_life == 0 && opt.diff != EMC_DIFF_TRUE_POISSON)
+        return fail(nullptr, EMC_E_INVALID, "emc_create: sigma_half_life must be non-zero");
+
+    emc_ctx *c = new emc_ctx();
+    c->dims = *d;
+    c->opt = opt;
+    c->L = d->model_side;
+    c->S = d->n_rotations;
+    c->I = d->n_patterns;
+    c->P = c->L * c->L;
+    c->N3 = (int64_t)c->L * c->L * c->L;
+    c->chunk = d->slice_chunk > 0 ? std::min(d->slice_chunk, c->S) : c->S;
+    c->chunkp = (int)round_up(c->chunk, 256);
+    c->mfma = opt.precision != EMC_PREC_FP32;
+    c->la_terms = opt.precision == EMC_PREC_BF16X3 ? 3 : 2;
+    const int nch = (c->S + c->chunk - 1) / c->chunk;
+    c->Sp = (int)round_up(std::max<int64_t>((int64_t)(nch - 1) * c->chunk + c->chunkp, c->S), kPadS);
+    c->Ip = (int)round_up(c->I, kPadI);
+    const int PpMax = (int)round_up(c->P, kPadP);
+
+    emc_status st = EMC_OK;
+    auto A = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && st == EMC_OK)
+            st = fail(c, e == hipErrorOutOfMemory ? EMC_E_OOM : EMC_E_HIP, std::string("emc_create: ") + what + ": " +
+                                                                             hipGetErrorString(e));
+    };
+    A(hipSetDevice(opt.device), "hipSetDevice");
+    if (st == EMC_OK) A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
+    for (auto &e : c->ev)
+        if (st == EMC_OK) A(hipEventCreate(&e), "event");
+    const int64_t S = c->S, Sp = c->Sp, Ip = c->Ip, P = c->P, N3 = c->N3, cp = c->chunkp;
+    if (st == EMC_OK) A(dalloc(&c->q, 4 * S), "q");
+    if (st == EMC_OK) A(dalloc(&c->rotm, 12 * Sp), "rotm");
+    if (st == EMC_OK) A(dalloc(&c->logw, Sp), "logw");
+    if (st == EMC_OK) A(dalloc(&c->coords, 3 * P), "coords");
+    if (st == EMC_OK) A(dalloc(&c->mask, P), "mask");
+    if (st == EMC_OK) A(dalloc(&c->axyz, 3 * (int64_t)PpMax), "axyz");
+    if (st == EMC_OK) A(dalloc(&c->pix, PpMax), "pix");
+    if (st == EMC_OK) A(dalloc(&c->sumK, Ip), "sumK");
+    if (c->mfma) {
+        if (st == EMC_OK) A(dalloc(&c->K16, Ip * PpMax), "K16");
+        if (st == EMC_OK) A(dalloc(&c->KT16, Ip * PpMax), "KT16");
+        if (st == EMC_OK) A(dalloc(&c->Lh, cp * PpMax), "Lh");
+        if (st == EMC_OK) A(dalloc(&c->Ll, cp * PpMax), "Ll");
+        if (st == EMC_OK && c->la_terms == 3) A(dalloc(&c->Lm, cp * PpMax), "Lm");
+        if (st == EMC_OK) A(dalloc(&c->Ph, Sp * Ip), "Ph");
+        if (st == EMC_OK) A(dalloc(&c->Pl, Sp * Ip), "Pl");
+        if (st == EMC_OK) A(dalloc(&c->colsum, PpMax), "colsum");
+    } else {
+        if (st == EMC_OK) A(dalloc(&c->K32, Ip * PpMax), "K32");
+        if (st == EMC_OK) A(dalloc(&c->KT32, Ip * PpMax), "KT32");
+        if (st == EMC_OK) A(dalloc(&c->A0, cp * PpMax), "A0");
+        if (st == EMC_OK && opt.diff != EMC_DIFF_TRUE_POISSON) A(dalloc(&c->A1, cp * PpMax), "A1");
+    }
+    if (st == EMC_OK) A(dalloc(&c->colmean, PpMax), "colmean");
+    if (st == EMC_OK) A(dalloc(&c->model, N3), "model");
+    if (st == EMC_OK) A(dalloc(&c->Mnum, N3), "Mnum");
+    if (st == EMC_OK) A(dalloc(&c->Wt, N3), "Wt");
+    if (st == EMC_OK) A(dalloc(&c->tmp3, N3), "tmp3");
+    if (st == EMC_OK) A(dalloc(&c->taps, c->L), "taps");
+    if (st == EMC_OK) A(dalloc(&c->sumW, Sp), "sumW");
+    if (st == EMC_OK) A(dalloc(&c->ns, Sp), "ns");
+    if (st == EMC_OK) A(dalloc(&c->bias, Sp), "bias");
+    if (st == EMC_OK) A(dalloc(&c->rscale, Sp), "rscale");
+    if (st == EMC_OK) A(dalloc(&c->cst, Ip), "cst");
+    if (st == EMC_OK) A(dalloc(&c->R, Sp * Ip), "R");
+    if (st == EMC_OK) A(dalloc(&c->lse, Ip), "lse");
+    c->part_len = std::max<int64_t>(emc::softmax_part_len(c->S, c->I), 4096);
+    if (st == EMC_OK) A(dalloc(&c->part, c->part_len), "part");
+    if (st == EMC_OK) A(dalloc(&c->T, Sp), "T");
+    if (st == EMC_OK && !(c->mfma && opt.insert == EMC_INSERT_FUSED)) A(dalloc(&c->U, cp * PpMax), "U");
+    if (st == EMC_OK) A(dalloc(&c->scal, SC_N), "scal");
+    if (st == EMC_OK) A(hipHostMalloc((void **)&c->scal_h, sizeof(double) * SC_N, hipHostMallocDefault), "scal_h");
+    if (st == EMC_OK && opt.blur_sigma > 0.0) {
+        std::vector<float> g = blur_taps(c->L, opt.blur_sigma);
+        A(hipMemcpy(c->taps, g.data(), sizeof(float) * c->L, hipMemcpyHostToDevice), "taps");
+    }
+    if (st != EMC_OK) {
+        std::string msg = c->err;
+        emc_destroy(c);
+        emc_host_fail(st, msg.c_str());
+        return st;
+    }
+    *out = c;
+    return EMC_OK;
+}
+
+// ---- inputs ----------------------------------------------------------------------------------
+
+extern "C" emc_status emc_set_rotations(emc_ctx *c, const float *q, const float *w)
+{
+    if (!c || !q || !w) return fail(c, EMC_E_INVALID, "set_rotations: null argument");
+    for (int s = 0; s < c->S; ++s) {
+        if (!(w[s] > 0.0f) || !std::isfinite(w[s])) return fail(c, EMC_E_INVALID, "set_rotations: weights must be > 0");
+        const double n2 = (double)q[4 * s] * q[4 * s] + (double)q[4 * s + 1] * q[4 * s + 1] +
+                          (double)q[4 * s + 2] * q[4 * s + 2] + (double)q[4 * s + 3] * q[4 * s + 3];
+        if (!(std::fabs(n2 - 1.0) < 1e-3)) return fail(c, EMC_E_INVALID, "set_rotations: quaternions must be unit");
+    }
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    float *wd = nullptr;
+    HIP_TRY(c, hipMalloc((void **)&wd, sizeof(float) * c->S));
+    hipError_t e = hipMemcpy(c->q, q, sizeof(float) * 4 * c->S, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(wd, w, sizeof(float) * c->S, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_rotmat(c->q, wd, c->S, c->Sp, c->rotm, c->logw, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(wd);
+    HIP_TRY(c, e);
+    c->has_rot = true;
+    c->has_logl = c->has_resp = false;
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_set_coords(emc_ctx *c, const float *x, const float *y, const float *z)
+{
+    if (!c || !x || !y || !z) return fail(c, EMC_E_INVALID, "set_coords: null argument");
+    std::vector<float> h((size_t)3 * c->P);
+    std::memcpy(h.data(), x, sizeof(float) * c->P);
+    std::memcpy(h.data() + c->P, y, sizeof(float) * c->P);
+    std::memcpy(h.data() + 2 * (size_t)c->P, z, sizeof(float) * c->P);
+    for (float v : h)
+        if (!std::isfinite(v)) return fail(c, EMC_E_INVALID, "set_coords: non-finite coordinate");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipMemcpy(c->coords, h.data(), sizeof(float) * 3 * c->P, hipMemcpyHostToDevice));
+    c->has_coords = true;
+    if (c->has_mask) {
+        std::vector<int32_t> m(c->P);
+        HIP_TRY(c, hipMemcpy(m.data(), c->mask, sizeof(int32_t) * c->P, hipMemcpyDeviceToHost));
+        EMC_TRY(rebuild_active(c, h.data(), m.data()));
+    }
+    c->has_logl = c->has_resp = false;
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_set_mask(emc_ctx *c, const int32_t *mask)
+{
+    if (!c || !mask) return fail(c, EMC_E_INVALID, "set_mask: null argument");
+    for (int p = 0; p < c->P; ++p)
+        if (mask[p] != 0 && mask[p] != 1) return fail(c, EMC_E_INVALID, "set_mask: values must be 0 or 1");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipMemcpy(c->mask, mask, sizeof(int32_t) * c->P, hipMemcpyHostToDevice));
+    std::vector<float> h;
+    if (c->has_coords) {
+        h.resize((size_t)3 * c->P);
+        HIP_TRY(c, hipMemcpy(h.data(), c->coords, sizeof(float) * 3 * c->P, hipMemcpyDeviceToHost));
+    }
+    EMC_TRY(rebuild_active(c, c->has_coords ? h.data() : nullptr, mask));
+    c->has_mask = true;
+    c->has_K = false;  // compaction changed
+    c->has_logl = c->has_resp = false;
+    return EMC_OK;
+}
+
+static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
+{
+    const int I = c->I, Ip = c->Ip, Pp = c->Pp, Pa = c->Pa, P = c->P;
+    std::vector<int32_t> pix(Pp);
+    HIP_TRY(c, hipMemcpy(pix.data(), c->pix, sizeof(int32_t) * Pp, hipMemcpyDeviceToHost));
+    std::vector<double> sumK(Ip, 0.0);
+    double total = 0.0, maxc = 0.0;
+    bool integral = true;
+    for (int i = 0; i < I; ++i)
+        for (int j = 0; j < Pa; ++j) {
+            const float k = Kf[(size_t)i * P + pix[j]];
+            if (!(k >= 0.0f) || !std::isfinite(k)) return fail(c, EMC_E_INVALID, "set_patterns: counts must be finite and >= 0");
+            sumK[i] += k;
+            maxc = std::max(maxc, (double)k);
+            if (k != std::floor(k)) integral = false;
+        }
+    for (int i = 0; i < I; ++i) total += sumK[i];
+    if (c->mfma) {
+        if (!integral) return fail(c, EMC_E_UNSUPPORTED, "set_patterns: MFMA precision modes need integer counts");
+        if (maxc > 65535.0) return fail(c, EMC_E_UNSUPPORTED, "set_patterns: counts above 65535 need precision = FP32");
+        c->k_split = maxc > 256.0;
+        std::vector<bf16_t> hi((size_t)Ip * Pp, 0), lo, hT((size_t)Ip * Pp, 0), lT;
+        if (c->k_split) {
+            lo.assign((size_t)Ip * Pp, 0);
+            lT.assign((size_t)Ip * Pp, 0);
+        }
+        for (int i = 0; i < I; ++i)
+            for (int j = 0; j < Pa; ++j) {
+                const float k = Kf[(size_t)i * P + pix[j]];
+                const bf16_t h = f2bf(k);
+                hi[(size_t)i * Pp + j] = h;
+                hT[(size_t)j * Ip + i] = h;
+                if (c->k_split) {
+                    const bf16_t l = f2bf(k - bf2f(h));
+                    lo[(size_t)i * Pp + j] = l;
+                    lT[(size_t)j * Ip + i] = l;
+                }
+            }
+        if (c->k_split && !c->K16lo) {
+            HIP_TRY(c, dalloc(&c->K16lo, (int64_t)Ip * round_up(P, kPadP)));
+            HIP_TRY(c, dalloc(&c->KT16lo, (int64_t)Ip * round_up(P, kPadP)));
+        }
+        HIP_TRY(c, hipMemcpy(c->K16, hi.data(), sizeof(bf16_t) * hi.size(), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->KT16, hT.data(), sizeof(bf16_t) * hT.size(), hipMemcpyHostToDevice));
+        if (c->k_split) {
+            HIP_TRY(c, hipMemcpy(c->K16lo, lo.data(), sizeof(bf16_t) * lo.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->KT16lo, lT.data(), sizeof(bf16_t) * lT.size(), hipMemcpyHostToDevice));
+        }
+    } else {
+        std::vector<float> k32((size_t)Ip * Pp, 0.0f), kt((size_t)Ip * Pp, 0.0f);
+        for (int i = 0; i < I; ++i)
+            for (int j = 0; j < Pa; ++j) {
+                const float k = Kf[(size_t)i * P + pix[j]];
+                k32[(size_t)i * Pp + j] = k;
+                kt[(size_t)j * Ip + i] = k;
+            }
+        HIP_TRY(c, hipMemcpy(c->K32, k32.data(), sizeof(float) * k32.size(), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(c->KT32, kt.data(), sizeof(float) * kt.size(), hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, hipMemcpy(c->sumK, sumK.data(), sizeof(double) * Ip, hipMemcpyHostToDevice));
+    c->sumK_total = total;
+    c->max_count = (int32_t)std::min(maxc, 2147483647.0);
+    c->has_K = true;
+    c->has_logl = c->has_resp = false;
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_set_patterns_f32(emc_ctx *c, const float *counts)
+{
+    if (!c || !counts) return fail(c, EMC_E_INVALID, "set_patterns: null argument");
+    if (!c->has_mask) return fail(c, EMC_E_STATE, "set_patterns: call emc_set_mask first");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    return set_patterns_impl(c, counts);
+}
+
+extern "C" emc_status emc_set_patterns_i32(emc_ctx *c, const int32_t *counts)
+{
+    if (!c || !counts) return fail(c, EMC_E_INVALID, "set_patterns: null argument");
+    if (!c->has_mask) return fail(c, EMC_E_STATE, "set_patterns: call emc_set_mask first");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    std::vector<float> f((size_t)c->I * c->P);
+    for (size_t k = 0; k < f.size(); ++k) {
+        if (counts[k] < 0) return fail(c, EMC_E_INVALID, "set_patterns: counts must be >= 0");
+        f[k] = (float)counts[k];
+    }
+    return set_patterns_impl(c, f.data());
+}
+
+extern "C" emc_status emc_set_model(emc_ctx *c, const float *model)
+{
+    if (!c || !model) return fail(c, EMC_E_INVALID, "set_model: null argument");
+    for (int64_t v = 0; v < c->N3; ++v)
+        if (!std::isfinite(model[v])) return fail(c, EMC_E_INVALID, "set_model: non-finite voxel");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipMemcpy(c->model, model, sizeof(float) * c->N3, hipMemcpyHostToDevice));
+    c->has_model = true;
+    c->has_logl = c->has_resp = false;
+    return EMC_OK;
+}
+
+// ---- outputs ---------------------------------------------------------------------------------
+
+extern "C" emc_status emc_get_model(emc_ctx *c, float *model, float *weight)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "get_model: null context");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (model) HIP_TRY(c, hipMemcpy(model, c->model, sizeof(float) * c->N3, hipMemcpyDeviceToHost));
+    if (weight) HIP_TRY(c, hipMemcpy(weight, c->Wt, sizeof(float) * c->N3, hipMemcpyDeviceToHost));
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_model_stats(emc_ctx *c, double *mx, double *avg)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "model_stats: null context");
+    if (!c->has_model) return fail(c, EMC_E_STATE, "model_stats: no model");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, launch_normalize(c, false, c->stream));
+    EMC_TRY(sync_scalars(c));
+    if (mx) *mx = c->scal_h[SC_MAX];
+    if (avg) *avg = c->scal_h[SC_AVG];
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_expand(emc_ctx *c, int32_t s0, int32_t count, float *out)
+{
+    if (!c || !out || s0 < 0 || count < 0 || s0 + count > c->S) return fail(c, EMC_E_INVALID, "expand: bad range");
+    if (!c->has_rot || !c->has_coords || !c->has_model) return fail(c, EMC_E_STATE, "expand: needs rotations, coords, model");
+    if (count == 0) return EMC_OK;
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    float *d = nullptr;
+    HIP_TRY(c, hipMalloc((void **)&d, sizeof(float) * (size_t)count * c->P));
+    hipError_t e = launch_expand(c, s0, count, d, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(float) * (size_t)count * c->P, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    HIP_TRY(c, e);
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_get_logl(emc_ctx *c, float *logits)
+{
+    if (!c || !logits) return fail(c, EMC_E_INVALID, "get_logl: null argument");
+    if (!c->has_logl) return fail(c, EMC_E_STATE, "get_logl: logits not available (run emc_logl; the FP32 path "
+                                                  "overwrites them in emc_normalize_resp)");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> cst(c->I);
+    HIP_TRY(c, hipMemcpy(cst.data(), c->cst, sizeof(double) * c->I, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy2D(logits, sizeof(float) * c->I, c->R, sizeof(float) * c->Ip, sizeof(float) * c->I, c->S,
+                           hipMemcpyDeviceToHost));
+    for (int s = 0; s < c->S; ++s)
+        for (int i = 0; i < c->I; ++i) {
+            float &v = logits[(size_t)s * c->I + i];
+            v = (float)((double)v + cst[i]);
+        }
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_get_resp(emc_ctx *c, float *resp)
+{
+    if (!c || !resp) return fail(c, EMC_E_INVALID, "get_resp: null argument");
+    if (!c->has_resp) return fail(c, EMC_E_STATE, "get_resp: no responsibilities");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (!c->mfma) {
+        HIP_TRY(c, hipMemcpy2D(resp, sizeof(float) * c->I, c->R, sizeof(float) * c->Ip, sizeof(float) * c->I, c->S,
+                               hipMemcpyDeviceToHost));
+        return EMC_OK;
+    }
+    std::vector<bf16_t> h((size_t)c->S * c->I), l((size_t)c->S * c->I);
+    HIP_TRY(c, hipMemcpy2D(h.data(), sizeof(bf16_t) * c->I, c->Ph, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * c->I,
+                           c->S, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy2D(l.data(), sizeof(bf16_t) * c->I, c->Pl, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * c->I,
+                           c->S, hipMemcpyDeviceToHost));
+    for (size_t k = 0; k < h.size(); ++k) resp[k] = bf2f(h[k]) + bf2f(l[k]);
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_get_lse(emc_ctx *c, double *lse)
+{
+    if (!c || !lse) return fail(c, EMC_E_INVALID, "get_lse: null argument");
+    if (!c->has_resp) return fail(c, EMC_E_STATE, "get_lse: no responsibilities");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    std::vector<double> cst(c->I);
+    HIP_TRY(c, hipMemcpy(lse, c->lse, sizeof(double) * c->I, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy(cst.data(), c->cst, sizeof(double) * c->I, hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->I; ++i) lse[i] += cst[i];
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_get_slice_weights(emc_ctx *c, double *T)
+{
+    if (!c || !T) return fail(c, EMC_E_INVALID, "get_slice_weights: null argument");
+    if (!c->has_resp) return fail(c, EMC_E_STATE, "get_slice_weights: no responsibilities");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    HIP_TRY(c, hipMemcpy(T, c->T, sizeof(double) * c->S, hipMemcpyDeviceToHost));
+    return EMC_OK;
+}
+
+// ---- stages ----------------------------------------------------------------------------------
+
+static emc_status logl_enqueue(emc_ctx *c, float sigma)
+{
+    hipStream_t st = c->stream;
+    ev_start(c, EMC_STAGE_SLICE);
+    HIP_TRY(c, launch_slice_stats(c, c->mfma, st));
+    HIP_TRY(c, launch_bias(c, sigma, st));
+    HIP_TRY(c, launch_pattern_const(c, c->mfma, st));
+    ev_stop(c, EMC_STAGE_SLICE);
+    ev_start(c, EMC_STAGE_LOGL);
+    for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
+        const int cs = std::min(c->chunk, c->S - s0);
+        if (c->mfma) {
+            HIP_TRY(c, launch_slice_bf16(c, s0, cs, st));
+            HIP_TRY(c, launch_logl_mfma(c, s0, cs, st));
+        } else {
+            HIP_TRY(c, launch_slice_f32(c, s0, cs, st));
+            HIP_TRY(c, launch_logl_f32(c, s0, cs, st));
+        }
+    }
+    ev_stop(c, EMC_STAGE_LOGL);
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_logl(emc_ctx *c, float sigma)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "logl: null context");
+    if (!c->has_rot || !c->has_coords || !c->has_mask || !c->has_K || !c->has_model)
+        return fail(c, EMC_E_STATE, "logl: needs rotations, coords, mask, patterns and model");
+    if (c->opt.diff != EMC_DIFF_TRUE_POISSON && !(sigma > 0.0f)) return fail(c, EMC_E_INVALID, "logl: sigma must be > 0");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
+    EMC_TRY(logl_enqueue(c, sigma));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->has_logl = true;
+    c->has_resp = false;
+    return EMC_OK;
+}
+
+static emc_status softmax_enqueue(emc_ctx *c)
+{
+    ev_start(c, EMC_STAGE_SOFTMAX);
+    HIP_TRY(c, launch_softmax(c, c->stream));
+    ev_stop(c, EMC_STAGE_SOFTMAX);
+    return EMC_OK;
+}
+
+static emc_status total_logl_host(emc_ctx *c, double *total)
+{
+    // sum_i lse_i = sum_i lse'_i + sum_i cst_i (FP64)
+    std::vector<double> cst(c->I);
+    HIP_TRY(c, hipMemcpy(cst.data(), c->cst, sizeof(double) * c->I, hipMemcpyDeviceToHost));
+    double t = c->scal_h[SC_TOTAL];
+    for (double v : cst) t += v;
+    *total = t;
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_normalize_resp(emc_ctx *c, double *total_logl)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "normalize_resp: null context");
+    if (!c->has_logl) return fail(c, EMC_E_STATE, "normalize_resp: run emc_logl first");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    EMC_TRY(softmax_enqueue(c));
+    EMC_TRY(sync_scalars(c));
+    c->has_resp = true;
+    if (!c->mfma) c->has_logl = false;  // P overwrote the logits
+    if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "normalize_resp: non-finite log-sum-exp");
+    if (total_logl) EMC_TRY(total_logl_host(c, total_logl));
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_set_resp(emc_ctx *c, const float *resp)
+{
+    if (!c || !resp) return fail(c, EMC_E_INVALID, "set_resp: null argument");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    const int S = c->S, I = c->I;
+    std::vector<double> T(c->Sp, 0.0);
+    for (int s = 0; s < S; ++s) {
+        double t = 0.0;
+        for (int i = 0; i < I; ++i) {
+            const float p = resp[(size_t)s * I + i];
+            if (!(p >= 0.0f) || !std::isfinite(p)) return fail(c, EMC_E_INVALID, "set_resp: values must be finite and >= 0");
+            t += p;
+        }
+        T[s] = t;
+    }
+    if (!c->mfma) {
+        HIP_TRY(c, hipMemcpy2D(c->R, sizeof(float) * c->Ip, resp, sizeof(float) * I, sizeof(float) * I, S,
+                               hipMemcpyHostToDevice));
+        c->has_logl = false;
+    } else {
+        std::vector<bf16_t> h((size_t)S * I), l((size_t)S * I);
+        for (size_t k = 0; k < h.size(); ++k) {
+            h[k] = f2bf(resp[k]);
+            l[k] = f2bf(resp[k] - bf2f(h[k]));
+        }
+        HIP_TRY(c, hipMemcpy2D(c->Ph, sizeof(bf16_t) * c->Ip, h.data(), sizeof(bf16_t) * I, sizeof(bf16_t) * I, S,
+                               hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy2D(c->Pl, sizeof(bf16_t) * c->Ip, l.data(), sizeof(bf16_t) * I, sizeof(bf16_t) * I, S,
+                               hipMemcpyHostToDevice));
+    }
+    HIP_TRY(c, hipMemcpy(c->T, T.data(), sizeof(double) * c->Sp, hipMemcpyHostToDevice));
+    c->has_resp = true;
+    return EMC_OK;
+}
+
+static emc_status mstep_enqueue(emc_ctx *c)
+{
+    hipStream_t st = c->stream;
+    ev_start(c, EMC_STAGE_MSTEP);
+    HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
+    HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
+    const bool fused = c->mfma && c->opt.insert == EMC_INSERT_FUSED;
+    for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
+        const int cs = std::min(c->chunk, c->S - s0);
+        if (c->mfma) {
+            HIP_TRY(c, launch_mstep_mfma(c, s0, cs, fused, st));
+            if (!fused) HIP_TRY(c, launch_insert(c, s0, cs, st));
+        } else {
+            HIP_TRY(c, launch_mstep_f32(c, s0, cs, st));
+            HIP_TRY(c, launch_insert(c, s0, cs, st));
+        }
+    }
+    ev_stop(c, EMC_STAGE_MSTEP);
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_maximize(emc_ctx *c)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "maximize: null context");
+    if (!c->has_resp || !c->has_K || !c->has_rot || !c->has_mask) return fail(c, EMC_E_STATE, "maximize: needs responsibilities");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    EMC_TRY(mstep_enqueue(c));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->has_mstep = true;
+    return EMC_OK;
+}
+
+static emc_status compress_enqueue(emc_ctx *c)
+{
+    ev_start(c, EMC_STAGE_COMPRESS);
+    HIP_TRY(c, launch_divide(c, c->stream));
+    if (c->opt.blur_sigma > 0.0) HIP_TRY(c, launch_blur(c, c->stream));
+    HIP_TRY(c, launch_normalize(c, true, c->stream));
+    ev_stop(c, EMC_STAGE_COMPRESS);
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_compress(emc_ctx *c, double *model_avg)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "compress: null context");
+    if (!c->has_mstep) return fail(c, EMC_E_STATE, "compress: run emc_maximize first");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
+    EMC_TRY(compress_enqueue(c));
+    EMC_TRY(sync_scalars(c));
+    if (model_avg) *model_avg = c->scal_h[SC_AVG];
+    c->has_logl = c->has_resp = false;
+    if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "compress: no positive voxel to normalise");
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_normalize_model(emc_ctx *c, double *model_avg)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "normalize_model: null context");
+    if (!c->has_model) return fail(c, EMC_E_STATE, "normalize_model: no model");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
+    HIP_TRY(c, launch_normalize(c, true, c->stream));
+    EMC_TRY(sync_scalars(c));
+    if (model_avg) *model_avg = c->scal_h[SC_AVG];
+    c->has_logl = c->has_resp = false;
+    if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "normalize_model: no positive voxel");
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats *stats)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "iterate: null context");
+    if (!c->has_rot || !c->has_coords || !c->has_mask || !c->has_K || !c->has_model)
+        return fail(c, EMC_E_STATE, "iterate: needs rotations, coords, mask, patterns and model");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    const double sigma = emc_sigma_schedule(iteration, c->opt.sigma_start, c->opt.sigma_final, c->opt.sigma_half_life);
+    if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL], c->stream);
+    HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
+    EMC_TRY(logl_enqueue(c, (float)sigma));
+    EMC_TRY(softmax_enqueue(c));
+    EMC_TRY(mstep_enqueue(c));
+    EMC_TRY(compress_enqueue(c));
+    if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL + 1], c->stream);
+    EMC_TRY(sync_scalars(c));
+    c->has_logl = c->mfma;
+    c->has_resp = true;
+    c->has_mstep = true;
+    if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "iterate: non-finite log-sum-exp or model normaliser");
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->sigma = sigma;
+        stats->fluence = c->scal_h[SC_PHI0];
+        stats->model_avg = c->scal_h[SC_AVG];
+        stats->weight_sum = c->scal_h[SC_WSUM];
+        EMC_TRY(total_logl_host(c, &stats->total_logl));
+        if (c->opt.timing) {
+            const int stages[] = {EMC_STAGE_SLICE, EMC_STAGE_LOGL, EMC_STAGE_SOFTMAX, EMC_STAGE_MSTEP, EMC_STAGE_COMPRESS,
+                                  EMC_STAGE_TOTAL};
+            for (int s : stages) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]) == hipSuccess) stats->stage_ms[s] = ms;
+            }
+        }
+    }
+    return EMC_OK;
+}

@@ -1,0 +1,269 @@
+// MFMA contractions for CDNA4 (gfx950): C[m, n] = sum_k sum_{(a,b)} A_a[m, k] B_b[n, k]
+// with A and B given as BF16 terms whose sum carries the operand (A_0 = hi, A_1 = next 8 bits, ...)
+// and FP32 accumulation in the matrix cores.
+//
+//   logL   : A = centred log-slices (NA = 2 or 3 terms) [rows = rotations], B = counts
+//            [rows = patterns] (NB = 1, or 2 when a count exceeds 256), k = active pixels.
+//            Epilogue: R[s, i] = acc + bias_s.
+//   M-step : A = responsibilities hi+lo [rows = rotations], B = counts^T [rows = pixels],
+//            k = patterns.  Epilogue: store U, or scatter U straight into the model (fused).
+//
+// Term pairs (a, b) with a + b < NA are accumulated (the dropped pairs are below 2^-8NA of the
+// product).  Tiling: 128 x 128 macro tile, 256 threads = 4 wavefronts of 64 lanes, each wave a
+// 64 x 64 block of 2 x 2 v_mfma_f32_32x32x16_bf16 tiles; BK = 64; operands staged in LDS with
+// rows padded to 72 BF16 (144 B) so the 16-byte fragment reads of 16 lanes hit 64 distinct
+// banks; the next k-tile is fetched into registers while the current one is multiplied.
+// The fused-scatter epilogue stages the accumulators through LDS, 64 rows at a time.
+#include "../emc_internal.h"
+#include "geom.h"
+
+namespace emc {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128, BK = 64, LDK = BK + 8;
+
+enum { EPI_LOGL = 0, EPI_STORE = 1, EPI_SCATTER = 2 };
+
+struct EpiArgs {
+    float *C;            // logits / U output
+    int64_t ldc;
+    int m_valid, n_valid;
+    const float *bias;   // [rows] (logL)
+    // fused scatter
+    const double *T;     // [rows] slice sums
+    const float *rotm;   // [rows][12]
+    const float *axyz;   // [3][Pp]
+    int Pp, Pa, L, interp;
+    float *Mnum, *Wt;
+};
+
+__device__ __forceinline__ void scatter_one(const EpiArgs &e, const float *m, float Ts, int j, float u)
+{
+    if (!(u >= 0.0f) || j >= e.Pa) return;
+    float ux, uy, uz;
+    rot_point(m, e.axyz[j], e.axyz[e.Pp + j], e.axyz[2 * e.Pp + j], model_c(e.L), ux, uy, uz);
+    const int L = e.L;
+    if (e.interp == EMC_INTERP_NEAREST) {
+        const int vx = nearest(ux), vy = nearest(uy), vz = nearest(uz);
+        if (vx >= 0 && vx < L && vy >= 0 && vy < L && vz >= 0 && vz < L) {
+            const size_t v = ((size_t)vz * L + vy) * L + vx;
+            atomicAdd(e.Mnum + v, u);
+            atomicAdd(e.Wt + v, Ts);
+        }
+    } else {
+        int bx, by, bz;
+        float fx, fy, fz;
+        if (!tri_axis(ux, L, bx, fx) || !tri_axis(uy, L, by, fy) || !tri_axis(uz, L, bz, fz)) return;
+        const float g[2][3] = {{1.0f - fx, 1.0f - fy, 1.0f - fz}, {fx, fy, fz}};
+        const size_t base = ((size_t)bz * L + by) * L + bx;
+        for (int dz = 0; dz < 2; ++dz)
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx) {
+                    const float w = g[dx][0] * g[dy][1] * g[dz][2];
+                    const size_t v = base + ((size_t)dz * L + dy) * L + dx;
+                    atomicAdd(e.Mnum + v, w * u);
+                    atomicAdd(e.Wt + v, w * Ts);
+                }
+    }
+}
+
+template <int NA, int NB, int EPI>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                                   const bf16_t *__restrict__ A2, int64_t lda,
+                                                   const bf16_t *__restrict__ B0, const bf16_t *__restrict__ B1,
+                                                   int64_t ldb, int Kd, EpiArgs e)
+{
+    constexpr int kTileBytes = (NA + NB) * BM * LDK * 2;
+    static_assert(kTileBytes >= 64 * (BN + 4) * 4, "LDS too small for the epilogue staging");
+    __shared__ __attribute__((aligned(16))) unsigned char smem[kTileBytes];
+    bf16_t(*As)[BM][LDK] = reinterpret_cast<bf16_t(*)[BM][LDK]>(smem);
+    bf16_t(*Bs)[BN][LDK] = reinterpret_cast<bf16_t(*)[BN][LDK]>(smem + NA * BM * LDK * 2);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+    // 128 rows x 64 k x 2 B = 1024 chunks of 16 B per operand term; 4 per thread
+    const int lrow = tid >> 3, lcol = (tid & 7) * 8;
+    const bf16_t *ga[NA], *gb[NB];
+    ga[0] = A0 + (size_t)(m0 + lrow) * lda + lcol;
+    if (NA > 1) ga[1] = A1 + (size_t)(m0 + lrow) * lda + lcol;
+    if (NA > 2) ga[NA - 1] = A2 + (size_t)(m0 + lrow) * lda + lcol;
+    gb[0] = B0 + (size_t)(n0 + lrow) * ldb + lcol;
+    if (NB > 1) gb[NB - 1] = B1 + (size_t)(n0 + lrow) * ldb + lcol;
+
+    u32x4 ra[NA][4], rb[NB][4];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) ra[a][h] = *reinterpret_cast<const u32x4 *>(ga[a] + (size_t)32 * h * lda);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) rb[b][h] = *reinterpret_cast<const u32x4 *>(gb[b] + (size_t)32 * h * ldb);
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int nk = Kd / BK;
+    const int fr = lane & 31, fk = (lane >> 5) * 8;
+    for (int kt = 0; kt < nk; ++kt) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) *reinterpret_cast<u32x4 *>(&As[a][lrow + 32 * h][lcol]) = ra[a][h];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int h = 0; h < 4; ++h) *reinterpret_cast<u32x4 *>(&Bs[b][lrow + 32 * h][lcol]) = rb[b][h];
+        __syncthreads();
+        if (kt + 1 < nk) {
+            const int k1 = (kt + 1) * BK;
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+#pragma unroll
+                for (int h = 0; h < 4; ++h)
+                    ra[a][h] = *reinterpret_cast<const u32x4 *>(ga[a] + (size_t)32 * h * lda + k1);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int h = 0; h < 4; ++h)
+                    rb[b][h] = *reinterpret_cast<const u32x4 *>(gb[b] + (size_t)32 * h * ldb + k1);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            bf16x8 fa[NA][2], fb[NB][2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                    fa[a][t] = *reinterpret_cast<const bf16x8 *>(&As[a][wm + t * 32 + fr][kk + fk]);
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    fb[b][t] = *reinterpret_cast<const bf16x8 *>(&Bs[b][wn + t * 32 + fr][kk + fk]);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int a = NA - 1; a >= 0; --a)
+#pragma unroll
+                        for (int b = NB - 1; b >= 0; --b)
+                            if (a + b < NA)
+                                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a][i], fb[b][j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+
+    // accumulator layout: lane holds column n = lane%32 of each 32x32 tile and rows
+    // 8*(r/4) + 4*(lane/32) + r%4, r = 0..15
+    if (EPI != EPI_SCATTER) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m >= e.m_valid) continue;
+                const float add = EPI == EPI_LOGL ? e.bias[m] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int n = n0 + wn + j * 32 + (lane & 31);
+                    if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = acc[i][j][r] + add;
+                }
+            }
+        return;
+    }
+
+    // fused scatter: stage one 64-row half of the tile in LDS, then every thread scatters a
+    // strided share of it (one compact loop instead of 64 unrolled scatter bodies)
+    float(*Cs)[BN + 4] = reinterpret_cast<float(*)[BN + 4]>(smem);
+    for (int half = 0; half < 2; ++half) {
+        if ((wave >> 1) == half) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        Cs[i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)][wn + j * 32 + (lane & 31)] = acc[i][j][r];
+        }
+        __syncthreads();
+        for (int row = wave; row < 64; row += 4) {
+            const int m = m0 + half * 64 + row;
+            if (m >= e.m_valid) continue;
+            const double Td = e.T[m];
+            if (!(Td > 1e-10)) continue;
+            float rm[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) rm[k] = e.rotm[12 * (size_t)m + k];
+            for (int col = lane; col < BN; col += 64) scatter_one(e, rm, (float)Td, n0 + col, Cs[row][col]);
+        }
+        __syncthreads();
+    }
+}
+
+template <int NA, int NB, int EPI>
+hipError_t run(dim3 grid, hipStream_t st, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2, int64_t lda,
+               const bf16_t *B0, const bf16_t *B1, int64_t ldb, int Kd, const EpiArgs &e)
+{
+    hipLaunchKernelGGL((k_gemm_mfma<NA, NB, EPI>), grid, dim3(256), 0, st, A0, A1, A2, lda, B0, B1, ldb, Kd, e);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
+{
+    EpiArgs e{};
+    e.C = c->R + (size_t)s0 * c->Ip;
+    e.ldc = c->Ip;
+    e.m_valid = cs;
+    e.n_valid = c->I;
+    e.bias = c->bias + s0;
+    const dim3 grid(c->Ip / BN, c->chunkp / BM);
+    const bf16_t *A1 = c->la_terms == 3 ? c->Lm : c->Ll;
+    const bf16_t *A2 = c->Ll;
+    if (c->la_terms == 3) {
+        if (c->k_split) return run<3, 2, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, c->K16lo, c->Pp, c->Pp, e);
+        return run<3, 1, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);
+    }
+    if (c->k_split) return run<2, 2, EPI_LOGL>(grid, st, c->Lh, A1, nullptr, c->Pp, c->K16, c->K16lo, c->Pp, c->Pp, e);
+    return run<2, 1, EPI_LOGL>(grid, st, c->Lh, A1, nullptr, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);
+}
+
+hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipStream_t st)
+{
+    EpiArgs e{};
+    e.C = c->U;
+    e.ldc = c->Pp;
+    e.m_valid = fused ? cs : c->chunkp;
+    e.n_valid = c->Pp;
+    e.T = c->T + s0;
+    e.rotm = c->rotm + 12 * (size_t)s0;
+    e.axyz = c->axyz;
+    e.Pp = c->Pp;
+    e.Pa = c->Pa;
+    e.L = c->L;
+    e.interp = c->opt.interp;
+    e.Mnum = c->Mnum;
+    e.Wt = c->Wt;
+    const dim3 grid(c->Pp / BN, c->chunkp / BM);
+    const bf16_t *Ah = c->Ph + (size_t)s0 * c->Ip, *Al = c->Pl + (size_t)s0 * c->Ip;
+    if (fused) {
+        if (c->k_split) return run<2, 2, EPI_SCATTER>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, c->KT16lo, c->Ip, c->Ip, e);
+        return run<2, 1, EPI_SCATTER>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);
+    }
+    if (c->k_split) return run<2, 2, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, c->KT16lo, c->Ip, c->Ip, e);
+    return run<2, 1, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);
+}
+
+}  // namespace emc

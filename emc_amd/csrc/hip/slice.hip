@@ -1,0 +1,427 @@
+// Expand: quaternion slicing of the 3D model (emc_cuda.cu:102-157) and the per-rotation
+// quantities the likelihood needs.
+//
+// Layout: active (mask != 0) pixels are compacted to j in [0, Pa) with coordinates axyz[3][Pp];
+// slices are written row-per-rotation [chunkp][Pp] so that both contractions read them with
+// the reduction index contiguous.  One workgroup owns one rotation row: the rotation matrix is
+// wave-uniform, coordinate reads are coalesced, and model reads are gathers that hit L2
+// (the 64^3 model is 1 MiB).
+#include <algorithm>
+
+#include "../emc_internal.h"
+#include "geom.h"
+
+namespace emc {
+namespace {
+
+constexpr int kStatsG = 16;  // rotations per workgroup in the statistics pass
+
+// Slice value of rotation matrix m at detector point (X,Y,Z); -1 when invalid.
+template <int INTERP>
+__device__ __forceinline__ float sample(const float *__restrict__ model, int L, const float *m,
+                                        float X, float Y, float Z)
+{
+    float ux, uy, uz;
+    rot_point(m, X, Y, Z, model_c(L), ux, uy, uz);
+    if (INTERP == EMC_INTERP_NEAREST) {
+        // lroundf and the exclusive lower bound of emc_cuda.cu:129-138
+        const int vx = nearest(ux), vy = nearest(uy), vz = nearest(uz);
+        if (vx > 0 && vx < L && vy > 0 && vy < L && vz > 0 && vz < L)
+            return model[((size_t)vz * L + vy) * L + vx];
+        return -1.0f;
+    } else {
+        int bx, by, bz;
+        float fx, fy, fz;
+        if (!tri_axis(ux, L, bx, fx) || !tri_axis(uy, L, by, fy) || !tri_axis(uz, L, bz, fz))
+            return -1.0f;
+        const float *b = model + ((size_t)bz * L + by) * L + bx;
+        const size_t LL = (size_t)L * L;
+        const float v000 = b[0], v001 = b[1], v010 = b[L], v011 = b[L + 1];
+        const float v100 = b[LL], v101 = b[LL + 1], v110 = b[LL + L], v111 = b[LL + L + 1];
+        const float mn = fminf(fminf(fminf(v000, v001), fminf(v010, v011)),
+                               fminf(fminf(v100, v101), fminf(v110, v111)));
+        if (mn < 0.0f) return -1.0f;  // a missing voxel among the corners
+        const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+        const float c00 = v000 * gx + v001 * fx, c01 = v010 * gx + v011 * fx;
+        const float c10 = v100 * gx + v101 * fx, c11 = v110 * gx + v111 * fx;
+        const float c0 = c00 * gy + c01 * fy, c1 = c10 * gy + c11 * fy;
+        return c0 * gz + c1 * fz;
+    }
+}
+
+__device__ __forceinline__ void load_rot(const float *__restrict__ rotm, int s, float *m)
+{
+    const float4 a = *reinterpret_cast<const float4 *>(rotm + 12 * (size_t)s);
+    const float4 b = *reinterpret_cast<const float4 *>(rotm + 12 * (size_t)s + 4);
+    const float c = rotm[12 * (size_t)s + 8];
+    m[0] = a.x; m[1] = a.y; m[2] = a.z; m[3] = a.w;
+    m[4] = b.x; m[5] = b.y; m[6] = b.z; m[7] = b.w; m[8] = c;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void k_rotmat(const float *__restrict__ q, const float *__restrict__ w, int S, int Sp,
+                         float *__restrict__ rotm, float *__restrict__ logw)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Sp) return;
+    float m[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float lw = 0.0f;
+    if (s < S) {
+        rotmat(q[4 * s], q[4 * s + 1], q[4 * s + 2], q[4 * s + 3], m);
+        lw = logf(w[s]);
+    }
+    for (int k = 0; k < 12; ++k) rotm[12 * (size_t)s + k] = m[k];
+    logw[s] = lw;
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ p, T v, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Per-rotation slice sum over V_s = {active j : W > 0}, |V_s|, and per-pixel sums of log W.
+// kStatsG rotations per workgroup; each thread walks pixels j = tid, tid+256, ... and keeps the
+// kStatsG partial sums in registers.
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ model, int L,
+                                                     const float *__restrict__ rotm, int S,
+                                                     const float *__restrict__ axyz, int Pa, int Pp,
+                                                     double *__restrict__ sumW, int32_t *__restrict__ ns,
+                                                     double *__restrict__ colsum)
+{
+    __shared__ float rm[kStatsG][9];
+    __shared__ double red_w[4][kStatsG];
+    __shared__ int red_n[4][kStatsG];
+    const int s0 = blockIdx.x * kStatsG;
+    const int tid = threadIdx.x;
+    if (tid < kStatsG * 9) {
+        const int g = tid / 9, k = tid % 9;
+        rm[g][k] = (s0 + g < S) ? rotm[12 * (size_t)(s0 + g) + k] : 0.0f;
+    }
+    __syncthreads();
+    const int ng = min(kStatsG, S - s0);
+    float sw[kStatsG];
+    int cnt[kStatsG];
+#pragma unroll
+    for (int g = 0; g < kStatsG; ++g) { sw[g] = 0.0f; cnt[g] = 0; }
+    for (int j = tid; j < Pa; j += 256) {
+        const float X = axyz[j], Y = axyz[Pp + j], Z = axyz[2 * Pp + j];
+        float col = 0.0f;
+#pragma unroll
+        for (int g = 0; g < kStatsG; ++g) {
+            if (g < ng) {
+                const float W = sample<INTERP>(model, L, rm[g], X, Y, Z);
+                if (W > 0.0f) {
+                    sw[g] += W;
+                    cnt[g] += 1;
+                    col += logf(W);
+                }
+            }
+        }
+        if (colsum) atomicAdd(colsum + j, (double)col);
+    }
+    const int wave = tid >> 6, lane = tid & 63;
+#pragma unroll
+    for (int g = 0; g < kStatsG; ++g) {
+        const double a = wave_sum((double)sw[g]);
+        const int b = wave_sum(cnt[g]);
+        if (lane == 0) { red_w[wave][g] = a; red_n[wave][g] = b; }
+    }
+    __syncthreads();
+    if (tid < ng) {
+        sumW[s0 + tid] = red_w[0][tid] + red_w[1][tid] + red_w[2][tid] + red_w[3][tid];
+        ns[s0 + tid] = red_n[0][tid] + red_n[1][tid] + red_n[2][tid] + red_n[3][tid];
+    }
+}
+
+// FP32 slice rows for the plain contraction.  KIND: TRUE_POISSON -> A0 = log W (0 if invalid);
+// chi^2 kinds -> A0 = W, A1 = pixel weight a (0 if invalid):
+//   poisson (reference chi^2) a = 1/(W+0.02), absolute a = 1, relative a = 1/W^2.
+template <int INTERP, int KIND>
+__global__ __launch_bounds__(256) void k_slice_f32(const float *__restrict__ model, int L,
+                                                   const float *__restrict__ rotm, int s0, int cs,
+                                                   const float *__restrict__ axyz, int Pa, int Pp,
+                                                   float *__restrict__ A0, float *__restrict__ A1)
+{
+    const int r = blockIdx.x;
+    const bool row_ok = r < cs;
+    float m[9];
+    if (row_ok) load_rot(rotm, s0 + r, m);
+    for (int j = threadIdx.x; j < Pp; j += 256) {
+        float a0 = 0.0f, a1 = 0.0f;
+        if (row_ok && j < Pa) {
+            const float W = sample<INTERP>(model, L, m, axyz[j], axyz[Pp + j], axyz[2 * Pp + j]);
+            if (W > 0.0f) {
+                if (KIND == EMC_DIFF_TRUE_POISSON) {
+                    a0 = logf(W);
+                } else {
+                    a0 = W;
+                    a1 = KIND == EMC_DIFF_POISSON ? 1.0f / (W + 0.02f)
+                                                  : (KIND == EMC_DIFF_ABSOLUTE ? 1.0f : 1.0f / (W * W));
+                }
+            }
+        }
+        A0[(size_t)r * Pp + j] = a0;
+        if (KIND != EMC_DIFF_TRUE_POISSON) A1[(size_t)r * Pp + j] = a1;
+    }
+}
+
+// Centred log-slice x = (W > 0 ? log W : 0) - colmean[j] split into NT BF16 terms
+// (hi, [mid,] lo) whose sum carries ~8*NT significant bits of x.
+template <int INTERP, int NT>
+__global__ __launch_bounds__(256) void k_slice_bf16(const float *__restrict__ model, int L,
+                                                    const float *__restrict__ rotm, int s0, int cs,
+                                                    const float *__restrict__ axyz, int Pa, int Pp,
+                                                    const float *__restrict__ colmean,
+                                                    bf16_t *__restrict__ Lh, bf16_t *__restrict__ Lm,
+                                                    bf16_t *__restrict__ Ll)
+{
+    const int r = blockIdx.x;
+    const bool row_ok = r < cs;
+    float m[9];
+    if (row_ok) load_rot(rotm, s0 + r, m);
+    for (int j = threadIdx.x; j < Pp; j += 256) {
+        float x = 0.0f;
+        if (row_ok && j < Pa) {
+            const float W = sample<INTERP>(model, L, m, axyz[j], axyz[Pp + j], axyz[2 * Pp + j]);
+            x = (W > 0.0f ? logf(W) : 0.0f) - colmean[j];
+        }
+        const bf16_t h = f2bf(x);
+        const float r1 = x - bf2f(h);
+        const bf16_t mi = f2bf(r1);
+        const size_t o = (size_t)r * Pp + j;
+        Lh[o] = h;
+        if (NT == 2) {
+            Ll[o] = mi;
+        } else {
+            Lm[o] = mi;
+            Ll[o] = f2bf(r1 - bf2f(mi));
+        }
+    }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_expand(const float *__restrict__ model, int L,
+                                                const float *__restrict__ rotm, int s0,
+                                                const float *__restrict__ coords, int P,
+                                                float *__restrict__ out)
+{
+    const int r = blockIdx.x;
+    float m[9];
+    load_rot(rotm, s0 + r, m);
+    for (int p = threadIdx.x; p < P; p += 256)
+        out[(size_t)r * P + p] = sample<INTERP>(model, L, m, coords[p], coords[P + p], coords[2 * P + p]);
+}
+
+// Fluence, per-rotation logit terms and pixel column means; one workgroup.
+//   TRUE_POISSON: phi0 = fluence or (sumK/I)/(sum_s sumW_s / S);
+//                 bias_s = log w_s - phi0 sumW_s + B0, B0 = phi0 * mean_s sumW_s (a global
+//                 shift that keeps the stored logits small; it is returned through cst).
+//   chi^2 kinds:  bias_s = log w_s, rscale_s = 1/(2 n_s sigma^2) (0 when n_s = 0).
+__global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, const int32_t *__restrict__ ns,
+                                               const float *__restrict__ logw, int S, int Sp, int I,
+                                               double sumK_total, double fluence, int diff, float sigma,
+                                               const double *__restrict__ colsum, int Pa, int Pp,
+                                               float *__restrict__ bias, float *__restrict__ rscale,
+                                               float *__restrict__ colmean, double *__restrict__ scal)
+{
+    __shared__ double red[16];
+    const int tid = threadIdx.x;
+    double acc = 0.0;
+    for (int s = tid; s < S; s += 1024) acc += sumW[s];
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) red[tid >> 6] = acc;
+    __syncthreads();
+    double SW = 0.0;
+    for (int k = 0; k < 16; ++k) SW += red[k];
+    double phi0 = 1.0;
+    if (diff == EMC_DIFF_TRUE_POISSON) {
+        if (fluence > 0.0) phi0 = fluence;
+        else if (SW > 0.0 && I > 0) phi0 = (sumK_total / I) / (SW / S);
+    }
+    const double B0 = (diff == EMC_DIFF_TRUE_POISSON && S > 0) ? phi0 * SW / S : 0.0;
+    if (tid == 0) {
+        scal[SC_PHI0] = phi0;
+        scal[SC_LOGPHI0] = log(phi0);
+        scal[SC_SUMW] = SW;
+    }
+    const double inv2s2 = 1.0 / (2.0 * (double)sigma * (double)sigma);
+    for (int s = tid; s < Sp; s += 1024) {
+        float b = 0.0f, rs = 0.0f;
+        if (s < S) {
+            if (diff == EMC_DIFF_TRUE_POISSON) b = (float)((double)logw[s] - phi0 * sumW[s] + B0);
+            else {
+                b = logw[s];
+                rs = ns[s] > 0 ? (float)(inv2s2 / (double)ns[s]) : 0.0f;
+            }
+        }
+        bias[s] = b;
+        rscale[s] = rs;
+    }
+    for (int j = tid; j < Pp; j += 1024)
+        colmean[j] = (colsum && j < Pa && S > 0) ? (float)(colsum[j] / S) : 0.0f;
+    if (tid == 0) scal[SC_B0] = B0;
+}
+
+// cst_i = sum_j K_ij colmean_j + sumK_i log(phi0) - B0   (FP64; one workgroup per pattern)
+__global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__ K32,
+                                                       const bf16_t *__restrict__ K16,
+                                                       const bf16_t *__restrict__ K16lo, int I, int Pp,
+                                                       const float *__restrict__ colmean,
+                                                       const double *__restrict__ sumK,
+                                                       const double *__restrict__ scal, bool centred,
+                                                       double *__restrict__ cst)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    double acc = 0.0;
+    if (centred) {
+        for (int j = threadIdx.x; j < Pp; j += 256) {
+            float k;
+            if (K32) k = K32[(size_t)i * Pp + j];
+            else {
+                k = bf2f(K16[(size_t)i * Pp + j]);
+                if (K16lo) k += bf2f(K16lo[(size_t)i * Pp + j]);
+            }
+            acc += (double)k * (double)colmean[j];
+        }
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        cst[i] = red[0] + red[1] + red[2] + red[3] + sumK[i] * scal[SC_LOGPHI0] - scal[SC_B0];
+}
+
+}  // namespace
+
+hipError_t launch_rotmat(const float *q, const float *w, int S, int Sp, float *rotm, float *logw,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rotmat, dim3((Sp + 255) / 256), dim3(256), 0, st, q, w, S, Sp, rotm, logw);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<float>, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<double>, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st)
+{
+    if (want_colsum) {
+        hipError_t e = launch_fill_f64(c->colsum, 0.0, c->Pp, st);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid((c->S + kStatsG - 1) / kStatsG);
+    double *cs = want_colsum ? c->colsum : nullptr;
+    if (c->opt.interp == EMC_INTERP_NEAREST)
+        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_NEAREST>, grid, dim3(256), 0, st, c->model, c->L,
+                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs);
+    else
+        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_TRILINEAR>, grid, dim3(256), 0, st, c->model, c->L,
+                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs);
+    return hipGetLastError();
+}
+
+hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_bias, dim3(1), dim3(1024), 0, st, c->sumW, c->ns, c->logw, c->S, c->Sp, c->I,
+                       c->sumK_total, c->opt.fluence, c->opt.diff, sigma, c->mfma ? c->colsum : nullptr,
+                       c->Pa, c->Pp, c->bias, c->rscale, c->colmean, c->scal);
+    return hipGetLastError();
+}
+
+hipError_t launch_pattern_const(const emc_ctx *c, bool centred, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pattern_const, dim3(c->I), dim3(256), 0, st, c->K32, c->K16, c->K16lo, c->I,
+                       c->Pp, c->colmean, c->sumK, c->scal, centred, c->cst);
+    return hipGetLastError();
+}
+
+template <int INTERP>
+static hipError_t slice_f32_interp(const emc_ctx *c, int s0, int cs, hipStream_t st)
+{
+    const dim3 grid(c->chunkp), blk(256);
+    switch (c->opt.diff) {
+    case EMC_DIFF_TRUE_POISSON:
+        hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_TRUE_POISSON>), grid, blk, 0, st, c->model, c->L,
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+        break;
+    case EMC_DIFF_POISSON:
+        hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_POISSON>), grid, blk, 0, st, c->model, c->L,
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+        break;
+    case EMC_DIFF_ABSOLUTE:
+        hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_ABSOLUTE>), grid, blk, 0, st, c->model, c->L,
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+        break;
+    default:
+        hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_RELATIVE>), grid, blk, 0, st, c->model, c->L,
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+        break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_slice_f32(const emc_ctx *c, int s0, int cs, hipStream_t st)
+{
+    return c->opt.interp == EMC_INTERP_NEAREST ? slice_f32_interp<EMC_INTERP_NEAREST>(c, s0, cs, st)
+                                               : slice_f32_interp<EMC_INTERP_TRILINEAR>(c, s0, cs, st);
+}
+
+hipError_t launch_slice_bf16(const emc_ctx *c, int s0, int cs, hipStream_t st)
+{
+    const dim3 grid(c->chunkp), blk(256);
+    const bool nn = c->opt.interp == EMC_INTERP_NEAREST;
+    if (c->la_terms == 3) {
+        if (nn)
+            hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_NEAREST, 3>), grid, blk, 0, st, c->model, c->L, c->rotm,
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+        else
+            hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_TRILINEAR, 3>), grid, blk, 0, st, c->model, c->L, c->rotm,
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+    } else {
+        if (nn)
+            hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_NEAREST, 2>), grid, blk, 0, st, c->model, c->L, c->rotm,
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+        else
+            hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_TRILINEAR, 2>), grid, blk, 0, st, c->model, c->L, c->rotm,
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_expand(const emc_ctx *c, int s0, int cs, float *out, hipStream_t st)
+{
+    if (cs <= 0) return hipSuccess;
+    if (c->opt.interp == EMC_INTERP_NEAREST)
+        hipLaunchKernelGGL(k_expand<EMC_INTERP_NEAREST>, dim3(cs), dim3(256), 0, st, c->model, c->L, c->rotm, s0,
+                           c->coords, c->P, out);
+    else
+        hipLaunchKernelGGL(k_expand<EMC_INTERP_TRILINEAR>, dim3(cs), dim3(256), 0, st, c->model, c->L, c->rotm,
+                           s0, c->coords, c->P, out);
+    return hipGetLastError();
+}
+
+}  // namespace emc

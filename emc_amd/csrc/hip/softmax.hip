@@ -1,0 +1,202 @@
+// Responsibilities: per-pattern log-sum-exp over rotations and P = exp(R - lse).
+//
+// Restates cuda_max_vector + cuda_respons_max_expf + the host log/add + cuda_norm_respons_sumexpf
+// (emc_cuda.cu:1041-1056, 1175-1217, 1253-1272; main.cpp:367-392) as three streaming passes over
+// the logits R[S][I] (row = rotation, so a pattern is a column):
+//   1. partial: each thread owns one pattern column over a band of rotation rows and keeps an
+//      online (max, sum of exp) pair; exp is FP32, the sum FP64.  Reads are coalesced rows.
+//   2. final:   combine the bands per pattern in FP64 -> lse'_i; per-block partial sums of lse'.
+//   3. apply:   P = exp(R - lse'_i) written either in place (FP32 path) or as a BF16 hi+lo pair
+//      (MFMA path), plus per-band row sums of P (T_s partials), reduced in FP64.
+// The reference's -1e10 floor on the max and on R - max is kept.
+#include <algorithm>
+
+#include "../emc_internal.h"
+
+namespace emc {
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_sm_partial(const float *__restrict__ R, int64_t ldr, int S, int I,
+                                                    int rows_per_band, float *__restrict__ pm,
+                                                    double *__restrict__ ps)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int band = blockIdx.y;
+    if (i >= I) return;
+    const int r0 = band * rows_per_band, r1 = min(S, r0 + rows_per_band);
+    float m = -1.0e10f;
+    double sum = 0.0;
+    const float *col = R + i;
+    for (int s = r0; s < r1; ++s) {
+        const float v = col[(size_t)s * ldr];
+        if (v > m) {
+            sum = sum * (double)expf(m - v) + 1.0;
+            m = v;
+        } else {
+            const float d = v - m;
+            if (d > -1.0e10f) sum += (double)expf(d);
+        }
+    }
+    pm[(size_t)band * I + i] = m;
+    ps[(size_t)band * I + i] = sum;
+}
+
+__global__ __launch_bounds__(256) void k_sm_final(const float *__restrict__ pm, const double *__restrict__ ps,
+                                                  int nbands, int I, double *__restrict__ lse,
+                                                  double *__restrict__ blk_sum, double *__restrict__ scal)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double l = 0.0;
+    if (i < I) {
+        float m = -1.0e10f;
+        for (int b = 0; b < nbands; ++b) m = fmaxf(m, pm[(size_t)b * I + i]);
+        double sum = 0.0;
+        for (int b = 0; b < nbands; ++b) {
+            const double sb = ps[(size_t)b * I + i];
+            if (sb > 0.0) sum += sb * exp((double)pm[(size_t)b * I + i] - (double)m);
+        }
+        l = (double)m + log(sum);
+        if (!isfinite(l)) scal[SC_NONFINITE] = 1.0;
+        lse[i] = l;
+    }
+    l = wave_sum(l);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_sum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// Deterministic single-workgroup sum of n doubles into *out.
+__global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, int n, double *__restrict__ out)
+{
+    __shared__ double red[16];
+    double a = 0.0;
+    for (int k = threadIdx.x; k < n; k += 1024) a += x[k];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < 16; ++k) t += red[k];
+        *out = t;
+    }
+}
+
+constexpr int kApplyRows = 8;    // rotation rows per workgroup
+constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
+
+// OUT = 0: P in place (FP32).  OUT = 1: BF16 hi/lo pair.
+template <int OUT>
+__global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t ldr, int S, int I,
+                                                  const double *__restrict__ lse, bf16_t *__restrict__ Ph,
+                                                  bf16_t *__restrict__ Pl, double *__restrict__ Tpart)
+{
+    __shared__ double red[4][kApplyRows];
+    const int c0 = blockIdx.x * kApplyCols + threadIdx.x * 4;
+    const int r0 = blockIdx.y * kApplyRows;
+    double l[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) l[k] = (c0 + k < I) ? lse[c0 + k] : 0.0;
+    double rowsum[kApplyRows];
+#pragma unroll
+    for (int rr = 0; rr < kApplyRows; ++rr) {
+        const int s = r0 + rr;
+        double acc = 0.0;
+        if (s < S && c0 < I) {
+            float *src = R + (size_t)s * ldr + c0;
+            const float4 v = *reinterpret_cast<const float4 *>(src);
+            const float x[4] = {v.x, v.y, v.z, v.w};
+            float p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                p[k] = (c0 + k < I) ? expf((float)((double)x[k] - l[k])) : 0.0f;
+                acc += (double)p[k];
+            }
+            if (OUT == 0) {
+                *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
+            } else {
+                bf16_t h[4], lo[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    h[k] = f2bf(p[k]);
+                    lo[k] = f2bf(p[k] - bf2f(h[k]));
+                }
+                const size_t o = (size_t)s * ldr + c0;
+                *reinterpret_cast<uint2 *>(Ph + o) =
+                    make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+                *reinterpret_cast<uint2 *>(Pl + o) =
+                    make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+            }
+        }
+        rowsum[rr] = acc;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int rr = 0; rr < kApplyRows; ++rr) {
+        const double t = wave_sum(rowsum[rr]);
+        if (lane == 0) red[wave][rr] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < kApplyRows) {
+        const int s = r0 + threadIdx.x;
+        const int k = threadIdx.x;
+        if (s < S) Tpart[(size_t)blockIdx.x * S + s] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    }
+}
+
+// T_s = sum over column blocks of the partial row sums (deterministic order).
+__global__ void k_T_final(const double *__restrict__ Tpart, int ncb, int S, int Sp, double *__restrict__ T)
+{
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= Sp) return;
+    double t = 0.0;
+    if (s < S)
+        for (int b = 0; b < ncb; ++b) t += Tpart[(size_t)b * S + s];
+    T[s] = t;
+}
+
+}  // namespace
+
+// Partials layout inside c->part:  pm (float, nb*I) | ps (double, nb*I) | blk (double) | Tpart.
+int64_t softmax_part_len(int S, int I)
+{
+    const int nb = std::max(1, std::min((S + 63) / 64, 4096 / std::max(1, (I + 255) / 256) + 1));
+    const int ncb = (I + kApplyCols - 1) / kApplyCols;
+    return (int64_t)nb * I /*pm as floats packed in doubles: over-allocate*/ + (int64_t)nb * I + (I + 255) / 256 + 1 +
+           (int64_t)ncb * S + 16;
+}
+
+hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
+{
+    const int S = c->S, I = c->I;
+    const int ncol = (I + 255) / 256;
+    const int nb = std::max(1, std::min((S + 63) / 64, 4096 / std::max(1, ncol) + 1));
+    const int rows = (S + nb - 1) / nb;
+    double *base = c->part;
+    float *pm = reinterpret_cast<float *>(base);
+    double *ps = base + (int64_t)nb * I;
+    double *blk = ps + (int64_t)nb * I;
+    double *Tpart = blk + ncol + 1;
+    hipLaunchKernelGGL(k_sm_partial, dim3(ncol, nb), dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, rows, pm, ps);
+    hipLaunchKernelGGL(k_sm_final, dim3(ncol), dim3(256), 0, st, pm, ps, nb, I, c->lse, blk, c->scal);
+    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, blk, ncol, c->scal + SC_TOTAL);
+    const int ncb = (I + kApplyCols - 1) / kApplyCols;
+    const dim3 ga(ncb, (S + kApplyRows - 1) / kApplyRows);
+    if (c->mfma)
+        hipLaunchKernelGGL(k_sm_apply<1>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, c->Ph, c->Pl,
+                           Tpart);
+    else
+        hipLaunchKernelGGL(k_sm_apply<0>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, nullptr, nullptr,
+                           Tpart);
+    hipLaunchKernelGGL(k_T_final, dim3((c->Sp + 255) / 256), dim3(256), 0, st, Tpart, ncb, S, c->Sp, c->T);
+    return hipGetLastError();
+}
+
+}  // namespace emc

@@ -1,0 +1,208 @@
+/*
+ * emc.h -- C ABI of the MI355X-native EMC engine (libemc.so).
+ *
+ * One context binds one HIP device and one stream and owns every device buffer.  Host pointers
+ * passed in are borrowed for the duration of the call only.  Every call returns an emc_status;
+ * emc_last_error() gives the message.  Nothing prints and nothing exits.
+ *
+ * Each entry point names the reference interface it replaces (JinLi971/EMC, files at the top
+ * level of that repository).  The reference exposes C++ free functions over raw device
+ * pointers (emc_cuda_help.h:9-117) driven by main.cpp:239-516; here the same stage granularity
+ * is kept (so each stage is testable on its own) but buffers live inside the context.
+ *
+ * Layouts at this boundary are the reference's: pixel index p = y*L + x; model index
+ * (z*L + y)*L + x; rotations [S][4] (q0,q1,q2,q3); weights [S]; patterns [I][L*L];
+ * responsibilities and logits [S][I] (row = rotation).
+ */
+#ifndef EMC_H
+#define EMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum emc_status {
+    EMC_OK = 0,
+    EMC_E_INVALID = 1,     /* bad argument or shape */
+    EMC_E_HIP = 2,         /* HIP runtime error */
+    EMC_E_OOM = 3,         /* device or host allocation failed */
+    EMC_E_NONFINITE = 4,   /* a non-finite value appeared where it must not (lse, sums) */
+    EMC_E_UNSUPPORTED = 5, /* valid request this build does not support */
+    EMC_E_STATE = 6,       /* call order violated (e.g. logl before set_model) */
+    EMC_E_IO = 7           /* file could not be read or written */
+} emc_status;
+
+/* configs.h:11  enum diff_type {absolute=0, poisson=1, relative=2}.  The reference's "poisson"
+ * is the chi^2 form sum (W-K)^2/(W+0.02) (emc_cuda.cu:196-216).  EMC_DIFF_TRUE_POISSON is the
+ * Poisson log-likelihood K.log W - sum W with one global fluence (no per-pattern scaling). */
+enum { EMC_DIFF_ABSOLUTE = 0, EMC_DIFF_POISSON = 1, EMC_DIFF_RELATIVE = 2, EMC_DIFF_TRUE_POISSON = 3 };
+
+/* Slicing/insertion interpolation.  NEAREST reproduces lroundf and the reference's bounds
+ * (emc_cuda.cu:129-134, emc_atomic.cu:62-69); TRILINEAR uses the 8 corners of [0, L-1]^3. */
+enum { EMC_INTERP_NEAREST = 0, EMC_INTERP_TRILINEAR = 1 };
+
+/* Arithmetic of the two contractions (logL and the M-step sum).
+ *   FP32   : direct FP32 (vector ALU), the plain path.
+ *   BF16X2 : MFMA, counts exact in BF16, log W and P each split into BF16 hi+lo, FP32 accumulate.
+ *   BF16X3 : as BF16X2 with log W split into three BF16 terms (hi+mid+lo).
+ * Counts above 256 are not exact in BF16: the MFMA modes then also split the counts into two
+ * exact BF16 terms.  Only EMC_DIFF_TRUE_POISSON runs on MFMA; the chi^2 kinds use FP32. */
+enum { EMC_PREC_FP32 = 0, EMC_PREC_BF16X2 = 1, EMC_PREC_BF16X3 = 2 };
+
+/* How the M-step inserts the weighted slice sums into the model.
+ *   ATOMIC : U = P^T.K is materialised per rotation chunk, then scattered with float atomics
+ *            (emc_atomic.cu:33-85, 183-200).
+ *   FUSED  : the MFMA M-step scatters from its accumulators (no U buffer). */
+enum { EMC_INSERT_ATOMIC = 0, EMC_INSERT_FUSED = 1 };
+
+typedef struct emc_dims {
+    int32_t model_side;  /* L: model L^3, patterns L x L (configs.h:14 model_side) */
+    int32_t n_rotations; /* S (rotations.c:457 return value) */
+    int32_t n_patterns;  /* I (configs.h:22 N_images) */
+    int32_t slice_chunk; /* rotations per chunk, 0 = all (configs.h:21 slice_chunk) */
+} emc_dims;
+
+typedef struct emc_options {
+    int32_t diff;            /* EMC_DIFF_*   (configs.h:52 diff) */
+    int32_t interp;          /* EMC_INTERP_* */
+    int32_t precision;       /* EMC_PREC_*   */
+    int32_t insert;          /* EMC_INSERT_* */
+    double sigma_start;      /* configs.h:47 */
+    double sigma_final;      /* configs.h:48 */
+    int32_t sigma_half_life; /* configs.h:49 */
+    int32_t device;          /* HIP device ordinal */
+    double blur_sigma;       /* model blur (voxels) after divide-by-weight; 0 = off (configs.h:80) */
+    double fluence;          /* TRUE_POISSON photons per model unit; 0 = automatic per iteration */
+    int32_t timing;          /* 1 = record per-stage HIP events (emc_iter_stats.stage_ms) */
+    int32_t reserved[7];
+} emc_options;
+
+enum {
+    EMC_STAGE_SLICE = 0,
+    EMC_STAGE_LOGL = 1,
+    EMC_STAGE_SOFTMAX = 2,
+    EMC_STAGE_MSTEP = 3,
+    EMC_STAGE_COMPRESS = 4,
+    EMC_STAGE_TOTAL = 5,
+    EMC_N_STAGES = 8
+};
+
+typedef struct emc_iter_stats {
+    double total_logl; /* sum_i lse_i, FP64 (the convergence measure) */
+    double sigma;      /* annealed width used (main.cpp:251) */
+    double fluence;    /* global fluence used (TRUE_POISSON) */
+    double model_avg;  /* mean of positive voxels before normalisation (emc_cuda.cu:639-661) */
+    double weight_sum; /* sum of the insertion weights Wt */
+    float stage_ms[EMC_N_STAGES]; /* HIP-event times when options.timing = 1 */
+} emc_iter_stats;
+
+typedef struct emc_ctx emc_ctx;
+
+/* ---- context ----------------------------------------------------------------------------- */
+
+/* Defaults: TRUE_POISSON, TRILINEAR, BF16X2, FUSED, sigma 0.4 -> 0.12 over 25 (emc.conf:8-10). */
+void emc_default_options(emc_options *opt);
+
+/* Replaces the allocation block of main.cpp:158-209 and cuda_set_device (emc_cuda.cu:866-869).
+ * Allocates every device buffer for dims; no allocation happens in later calls. */
+emc_status emc_create(const emc_dims *dims, const emc_options *opt, emc_ctx **out);
+void emc_destroy(emc_ctx *ctx);
+const char *emc_last_error(const emc_ctx *ctx); /* ctx may be NULL: last create/host error */
+emc_status emc_device_info(int32_t device, char *name, int32_t name_len, int32_t *n_cu,
+                           int64_t *hbm_bytes); /* cuda_print_device_info (emc_cuda.cu:800-810) */
+
+/* ---- inputs (host -> device) --------------------------------------------------------------- */
+
+/* cuda_allocate_rotations (emc_cuda.cu:695-701) + cuda_copy_weight_to_device (emc_cuda.cu:903-907).
+ * q: [S][4] unit quaternions, w: [S] weights (> 0). */
+emc_status emc_set_rotations(emc_ctx *ctx, const float *q, const float *w);
+/* cuda_allocate_coords (emc_cuda.cu:718-726).  Each [L*L], index y*L + x. */
+emc_status emc_set_coords(emc_ctx *ctx, const float *x, const float *y, const float *z);
+/* cuda_allocate_mask (emc_cuda.cu:689-693).  [L*L], values 0 or 1. */
+emc_status emc_set_mask(emc_ctx *ctx, const int32_t *mask);
+/* cuda_allocate_images (emc_cuda.cu:703-709).  [I][L*L] photon counts (>= 0).  Must follow
+ * emc_set_mask.  The i32 form is the exact-count path used by the MFMA modes. */
+emc_status emc_set_patterns_i32(emc_ctx *ctx, const int32_t *counts);
+emc_status emc_set_patterns_f32(emc_ctx *ctx, const float *counts);
+/* cuda_allocate_model / cuda_copy_model_2_device (emc_cuda.cu:675-685).  [L^3]. */
+emc_status emc_set_model(emc_ctx *ctx, const float *model);
+
+/* ---- outputs (device -> host) -------------------------------------------------------------- */
+
+/* cuda_copy_model (emc_cuda.cu:608-610), main.cpp:479-480.  Either pointer may be NULL. */
+emc_status emc_get_model(emc_ctx *ctx, float *model, float *weight);
+/* cuda_model_max / cuda_model_average (emc_cuda.cu:629-671). */
+emc_status emc_model_stats(emc_ctx *ctx, double *max, double *average);
+/* Logits of the last emc_logl, [S][I], constants included (the reference's d_respons before
+ * normalisation, emc_cuda.cu:270-275). */
+emc_status emc_get_logl(emc_ctx *ctx, float *logits);
+/* Responsibilities [S][I] after emc_normalize_resp (the isDebug dump, main.cpp:395-417). */
+emc_status emc_get_resp(emc_ctx *ctx, float *resp);
+/* Per-pattern log-sum-exp [I] (main.cpp:384-391, accumulated in FP64). */
+emc_status emc_get_lse(emc_ctx *ctx, double *lse);
+/* Slice sums of the last M-step: T [S] (emc_atomic.cu:114-122). */
+emc_status emc_get_slice_weights(emc_ctx *ctx, double *T);
+
+/* ---- stages (main.cpp:239-480) ------------------------------------------------------------- */
+
+/* cuda_get_slices (emc_cuda.cu:542-560): slices of rotations [s0, s0+count) into out
+ * [count][L*L]; invalid samples (and masked pixels) are -1. */
+emc_status emc_expand(emc_ctx *ctx, int32_t s0, int32_t count, float *out);
+/* Slicing + likelihood for all rotations: the chunk loop of main.cpp:351-365
+ * (cuda_get_slices + cuda_calculate_responsabilities, emc_cuda.cu:280-295). */
+emc_status emc_logl(emc_ctx *ctx, float sigma);
+/* cuda_max_vector + cuda_respons_max_expf + host log/add + cuda_norm_respons_sumexpf
+ * (main.cpp:367-392).  total_logl (may be NULL) receives sum_i lse_i. */
+emc_status emc_normalize_resp(emc_ctx *ctx, double *total_logl);
+/* Replaces the responsibilities (for stage tests / external E-steps): [S][I]. */
+emc_status emc_set_resp(emc_ctx *ctx, const float *resp);
+/* cuda_reset_model x2 + chunk loop of cuda_update_slices (main.cpp:443-462). */
+emc_status emc_maximize(emc_ctx *ctx);
+/* cuda_divide_model_by_weight + model blur + cuda_normalize_model (main.cpp:477-478). */
+emc_status emc_compress(emc_ctx *ctx, double *model_avg);
+/* cuda_normalize_model (emc_cuda.cu:788-798) on the current model (main.cpp:180). */
+emc_status emc_normalize_model(emc_ctx *ctx, double *model_avg);
+/* One EMC iteration, main.cpp:239-480: sigma schedule, logl, normalise, maximize, compress.
+ * stats may be NULL.  Synchronises before returning. */
+emc_status emc_iterate(emc_ctx *ctx, int32_t iteration, emc_iter_stats *stats);
+
+/* ---- host utilities (no GPU needed) -------------------------------------------------------- */
+
+/* n_to_samples(n)/2 (rotations.c:43, :457). */
+int32_t emc_rotation_count(int32_t n);
+/* generate_rotation_list (rotations.c:77-458): same order, FP32 rounding, pruning and weights.
+ * q [S][4], w [S] with S = emc_rotation_count(n). */
+emc_status emc_generate_rotations(int32_t n, float *q, float *w);
+/* emc_helper::calculate_coordinates (emc_helper.cpp:55-79). */
+emc_status emc_coords_ewald(int32_t L, float pixel_size, float detector_distance,
+                            float wavelength, float *x, float *y, float *z);
+/* Flat detector plane (Z = 0) of the synthetic workloads. */
+emc_status emc_coords_flat(int32_t L, float *x, float *y, float *z);
+/* Annular mask rb <= r <= ro on pixel-centre radius; returns the active count (< 0 on error). */
+int32_t emc_mask_annulus(int32_t L, double rb, double ro, int32_t *mask);
+/* Annealed width of main.cpp:251. */
+double emc_sigma_schedule(int32_t iteration, double sigma_start, double sigma_final,
+                          int32_t half_life);
+
+/* Synthetic data (no reference counterpart; the reference reads HDF5, emc_helper.cpp:198-248).
+ * Object: n_atoms Gaussian atoms (sigma 1 voxel, unit peak) uniform in a sphere of `radius`
+ * voxels on an L^3 grid.  Model: fftshifted |DFT|^2 of the object, mean 1. */
+emc_status emc_synth_object(int32_t L, int32_t n_atoms, double radius, uint64_t seed,
+                            float *object);
+emc_status emc_synth_model(int32_t L, int32_t n_atoms, double radius, uint64_t seed, float *model);
+/* Patterns: a seeded uniform random quaternion per pattern (Shoemake, as rotations.c:30-41),
+ * a trilinear slice of `model` at coords, scaled to `photons` expected counts over mask == 1
+ * pixels, Poisson-sampled.  counts [I][L*L] (0 on masked pixels), q_true [I][4] (may be NULL). */
+emc_status emc_synth_patterns(const float *model, int32_t L, const float *x, const float *y,
+                              const float *z, const int32_t *mask, int32_t n_patterns,
+                              double photons, uint64_t seed, int32_t *counts, float *q_true);
+/* Uniform random [0,1) model (model_input = 0, emc_helper.cpp:449-456, own RNG). */
+emc_status emc_synth_uniform_model(int32_t L, uint64_t seed, float *model);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMC_H */
