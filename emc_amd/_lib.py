@@ -21,7 +21,7 @@ INTERP = {"nearest": 0, "trilinear": 1}
 PRECISION = {"fp32": 0, "bf16x2": 1, "bf16x3": 2}
 INSERT = {"atomic": 0, "fused": 1}
 N_STAGES = 8
-STAGES = ["slice", "logl", "softmax", "mstep", "compress", "total"]
+STAGES = ["slice", "logl", "softmax", "mstep", "compress", "total", "logl_gemm", "mstep_gemm"]
 
 
 class Dims(C.Structure):

@@ -85,6 +85,24 @@ void ev_stop(emc_ctx *c, int stage)
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * stage + 1], c->stream);
 }
 
+// contraction-kernel events: kind 0 = logL, 1 = M-step; chunk index k
+void gev_rec(emc_ctx *c, int kind, int k, int edge)
+{
+    if (c->opt.timing) (void)hipEventRecord(c->gev[4 * (size_t)k + 2 * kind + edge], c->stream);
+}
+
+float gev_sum(emc_ctx *c, int kind)
+{
+    float tot = 0.0f;
+    const int nch = (int)(c->gev.size() / 4);
+    for (int k = 0; k < nch; ++k) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->gev[4 * (size_t)k + 2 * kind], c->gev[4 * (size_t)k + 2 * kind + 1]) == hipSuccess)
+            tot += ms;
+    }
+    return tot;
+}
+
 emc_status sync_scalars(emc_ctx *c)
 {
     HIP_TRY(c, hipMemcpyAsync(c->scal_h, c->scal, sizeof(double) * SC_N, hipMemcpyDeviceToHost, c->stream));
@@ -170,6 +188,8 @@ extern "C" void emc_destroy(emc_ctx *c)
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->gev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -217,6 +237,9 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     A(hipSetDevice(opt.device), "hipSetDevice");
     if (st == EMC_OK) A(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream");
     for (auto &e : c->ev)
+        if (st == EMC_OK) A(hipEventCreate(&e), "event");
+    c->gev.assign(4 * (size_t)nch, nullptr);
+    for (auto &e : c->gev)
         if (st == EMC_OK) A(hipEventCreate(&e), "event");
     const int64_t S = c->S, Sp = c->Sp, Ip = c->Ip, P = c->P, N3 = c->N3, cp = c->chunkp;
     if (st == EMC_OK) A(dalloc(&c->q, 4 * S), "q");
@@ -555,12 +578,17 @@ static emc_status logl_enqueue(emc_ctx *c, float sigma)
     ev_start(c, EMC_STAGE_LOGL);
     for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
         const int cs = std::min(c->chunk, c->S - s0);
+        const int k = s0 / c->chunk;
         if (c->mfma) {
             HIP_TRY(c, launch_slice_bf16(c, s0, cs, st));
+            gev_rec(c, 0, k, 0);
             HIP_TRY(c, launch_logl_mfma(c, s0, cs, st));
+            gev_rec(c, 0, k, 1);
         } else {
             HIP_TRY(c, launch_slice_f32(c, s0, cs, st));
+            gev_rec(c, 0, k, 0);
             HIP_TRY(c, launch_logl_f32(c, s0, cs, st));
+            gev_rec(c, 0, k, 1);
         }
     }
     ev_stop(c, EMC_STAGE_LOGL);
@@ -659,13 +687,12 @@ static emc_status mstep_enqueue(emc_ctx *c)
     const bool fused = c->mfma && c->opt.insert == EMC_INSERT_FUSED;
     for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
         const int cs = std::min(c->chunk, c->S - s0);
-        if (c->mfma) {
-            HIP_TRY(c, launch_mstep_mfma(c, s0, cs, fused, st));
-            if (!fused) HIP_TRY(c, launch_insert(c, s0, cs, st));
-        } else {
-            HIP_TRY(c, launch_mstep_f32(c, s0, cs, st));
-            HIP_TRY(c, launch_insert(c, s0, cs, st));
-        }
+        const int k = s0 / c->chunk;
+        gev_rec(c, 1, k, 0);
+        if (c->mfma) HIP_TRY(c, launch_mstep_mfma(c, s0, cs, fused, st));
+        else HIP_TRY(c, launch_mstep_f32(c, s0, cs, st));
+        gev_rec(c, 1, k, 1);
+        if (!fused) HIP_TRY(c, launch_insert(c, s0, cs, st));
     }
     ev_stop(c, EMC_STAGE_MSTEP);
     return EMC_OK;
@@ -753,6 +780,8 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
                 float ms = 0.0f;
                 if (hipEventElapsedTime(&ms, c->ev[2 * s], c->ev[2 * s + 1]) == hipSuccess) stats->stage_ms[s] = ms;
             }
+            stats->stage_ms[EMC_STAGE_LOGL_GEMM] = gev_sum(c, 0);
+            stats->stage_ms[EMC_STAGE_MSTEP_GEMM] = gev_sum(c, 1);
         }
     }
     return EMC_OK;

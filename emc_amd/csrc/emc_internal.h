@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/emc.h"
 
@@ -56,6 +57,7 @@ struct emc_ctx {
 
     hipStream_t stream = nullptr;
     hipEvent_t ev[2 * EMC_N_STAGES] = {};  // start/stop per stage
+    std::vector<hipEvent_t> gev;           // start/stop per chunk: logL and M-step contractions
 
     // geometry and inputs
     float *q = nullptr;       // [S][4]

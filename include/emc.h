@@ -79,6 +79,8 @@ typedef struct emc_options {
     int32_t reserved[7];
 } emc_options;
 
+/* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone
+ * (summed over rotation chunks); the other slots bracket whole stages. */
 enum {
     EMC_STAGE_SLICE = 0,
     EMC_STAGE_LOGL = 1,
@@ -86,6 +88,8 @@ enum {
     EMC_STAGE_MSTEP = 3,
     EMC_STAGE_COMPRESS = 4,
     EMC_STAGE_TOTAL = 5,
+    EMC_STAGE_LOGL_GEMM = 6,
+    EMC_STAGE_MSTEP_GEMM = 7,
     EMC_N_STAGES = 8
 };
 

@@ -414,35 +414,46 @@ ORC_API void orc_logl(const double *W, const double *K, const int32_t *mask, int
                       int diff, double sigma, const double *wrot, const double *phi, double phi0,
                       double *R, int32_t *n_s_out)
 {
-#pragma omp parallel for schedule(dynamic, 4)
-    for (int s = 0; s < S; s++) {
-        const double *Ws = W + (size_t)s * P;
-        int ns = 0;
-        double sumW = 0.0;
-        for (int p = 0; p < P; p++)
-            if (mask[p] != 0 && Ws[p] > 0.0) { ns++; sumW += Ws[p]; }
-        if (n_s_out) n_s_out[s] = ns;
-        const double lw = log(wrot[s]);
-        for (int i = 0; i < I; i++) {
-            const double *Ki = K + (size_t)i * P;
-            const double ph = phi ? phi[(size_t)s * I + i] : 1.0;
-            double acc = 0.0;
-            if (diff == 3) {
-                for (int p = 0; p < P; p++)
-                    if (mask[p] != 0 && Ws[p] > 0.0) acc += Ki[p] * log(phi0 * Ws[p]);
-                R[(size_t)s * I + i] = lw + acc - phi0 * sumW;
-            } else {
-                for (int p = 0; p < P; p++) {
-                    if (!(mask[p] != 0 && Ws[p] > 0.0)) continue;
-                    double d = Ws[p] - Ki[p] / ph;
-                    if (diff == 0) acc += d * d;
-                    else if (diff == 1) acc += d * d / (Ws[p] + 0.02);
-                    else acc += (d / Ws[p]) * (d / Ws[p]);
+#pragma omp parallel
+    {
+        double *lw = (double *)malloc(sizeof(double) * (size_t)P);
+        int *vp = (int *)malloc(sizeof(int) * (size_t)P);
+#pragma omp for schedule(dynamic, 4)
+        for (int s = 0; s < S; s++) {
+            const double *Ws = W + (size_t)s * P;
+            int ns = 0;
+            double sumW = 0.0;
+            for (int p = 0; p < P; p++)
+                if (mask[p] != 0 && Ws[p] > 0.0) {
+                    vp[ns++] = p;
+                    sumW += Ws[p];
                 }
-                double t = ns > 0 ? acc / 2.0 / (double)ns / (sigma * sigma) : 0.0;
-                R[(size_t)s * I + i] = lw - t;
+            if (n_s_out) n_s_out[s] = ns;
+            const double lws = log(wrot[s]);
+            if (diff == 3)
+                for (int k = 0; k < ns; k++) lw[k] = log(phi0 * Ws[vp[k]]);
+            for (int i = 0; i < I; i++) {
+                const double *Ki = K + (size_t)i * P;
+                const double ph = phi ? phi[(size_t)s * I + i] : 1.0;
+                double acc = 0.0;
+                if (diff == 3) {
+                    for (int k = 0; k < ns; k++) acc += Ki[vp[k]] * lw[k];
+                    R[(size_t)s * I + i] = lws + acc - phi0 * sumW;
+                } else {
+                    for (int k = 0; k < ns; k++) {
+                        const int p = vp[k];
+                        double d = Ws[p] - Ki[p] / ph;
+                        if (diff == 0) acc += d * d;
+                        else if (diff == 1) acc += d * d / (Ws[p] + 0.02);
+                        else acc += (d / Ws[p]) * (d / Ws[p]);
+                    }
+                    double t = ns > 0 ? acc / 2.0 / (double)ns / (sigma * sigma) : 0.0;
+                    R[(size_t)s * I + i] = lws - t;
+                }
             }
         }
+        free(lw);
+        free(vp);
     }
 }
 
@@ -525,15 +536,16 @@ ORC_API void orc_mstep(const double *Presp, const double *K, const int32_t *mask
             double T = 0.0;
             for (int i = 0; i < I; i++) T += Presp[(size_t)s * I + i];
             if (Tout) Tout[s] = T;
-            for (int p = 0; p < P; p++) {
-                if (mask[p] == 0) { U[p] = -1.0; continue; }
-                double acc = 0.0;
-                for (int i = 0; i < I; i++) {
-                    double ph = phi ? phi[(size_t)s * I + i] : 1.0;
-                    acc += K[(size_t)i * P + p] * Presp[(size_t)s * I + i] / ph;
-                }
-                U[p] = acc;
+            for (int p = 0; p < P; p++) U[p] = 0.0;
+            for (int i = 0; i < I; i++) {
+                const double pr = Presp[(size_t)s * I + i];
+                if (pr == 0.0) continue;
+                const double a = pr / (phi ? phi[(size_t)s * I + i] : 1.0);
+                const double *Ki = K + (size_t)i * P;
+                for (int p = 0; p < P; p++) U[p] += Ki[p] * a;
             }
+            for (int p = 0; p < P; p++)
+                if (mask[p] == 0) U[p] = -1.0;
             if (!(T > 1e-10)) continue;
             for (int p = 0; p < P; p++)
                 if (mask[p] != 0) U[p] /= T;

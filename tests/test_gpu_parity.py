@@ -1,0 +1,237 @@
+"""GPU parity: each stage and whole iterations of libemc against the FP64 oracle, through the
+C ABI.  Tolerances are written per test; the discrete decisions (validity, voxel indices) are
+bit-identical by construction and compared exactly."""
+import numpy as np
+import pytest
+
+import emc_amd
+from emc_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def metrics(R_gpu, R_orc, P_gpu, P_orc, M_gpu, M_orc):
+    out = {}
+    if R_gpu is not None:
+        out["rel_logl"] = float(np.max(np.abs(R_gpu.astype(np.float64) - R_orc) / np.maximum(np.abs(R_orc), 1.0)))
+    if P_gpu is not None:
+        l1 = np.abs(P_gpu.astype(np.float64) - P_orc).sum(0)
+        out["resp_l1_max"] = float(l1.max())
+        out["resp_l1_mean"] = float(l1.mean())
+    if M_gpu is not None:
+        out["model_rel_l2"] = float(np.linalg.norm(M_gpu - M_orc) / np.linalg.norm(M_orc))
+    return out
+
+
+@pytest.fixture(scope="module")
+def small():
+    """32^3 model, 3,240 rotations, 1,000 patterns (the full-size small workload)."""
+    d = wl.make("L32_S3240_I1000")
+    d["m0"] = wl.normalized(d["model0"])
+    return d
+
+
+def engine(d, prec="fp32", S=None, I=None, **kw):
+    L = d["L"]
+    S = d["q"].shape[0] if S is None else S
+    I = d["K"].shape[0] if I is None else I
+    e = emc_amd.Engine(L, S, I, precision=prec, **kw)
+    e.setup(d["q"][:S], d["w"][:S], d["coords"], d["mask"], d["K"][:I], d["m0"].astype(np.float32))
+    return e
+
+
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_expand_matches_oracle(orc, small, interp):
+    log(f"[expand] interp={interp}")
+    d = small
+    with engine(d, "fp32", interp=interp) as e:
+        W = e.expand(100, 200)
+    Wo = orc.slices(d["m0"].astype(np.float32).astype(np.float64), d["L"], d["q"][100:300], d["coords"], interp)
+    assert np.array_equal(W == -1, Wo == -1)            # identical validity decisions
+    v = Wo != -1
+    assert np.allclose(W[v], Wo[v], rtol=2e-6, atol=0)
+
+
+@pytest.mark.parametrize("diff", ["true_poisson", "poisson", "absolute", "relative"])
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_logl_fp32_matches_oracle(orc, small, diff, interp):
+    log(f"[logl fp32] diff={diff} interp={interp}")
+    d = small
+    S, I = 700, 300
+    sigma = 0.3
+    with engine(d, "fp32", S=S, I=I, diff=diff, interp=interp) as e:
+        e.logl(sigma)
+        R = e.get_logl()
+    M32 = d["m0"].astype(np.float32).astype(np.float64)
+    W = orc.slices(M32, d["L"], d["q"][:S], d["coords"], interp)
+    K = d["K"][:I].astype(np.float64)
+    phi0 = orc.fluence(W, K, d["mask"]) if diff == "true_poisson" else 1.0
+    Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), diff, sigma=sigma, phi0=phi0)
+    err = np.abs(R - Ro) / np.maximum(np.abs(Ro), 1.0)
+    log(f"   max rel err {err.max():.2e}")
+    assert err.max() < 2e-5
+
+
+@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
+def test_logl_mfma_matches_oracle(orc, small, prec):
+    log(f"[logl mfma] prec={prec}")
+    d = small
+    S, I = 1000, 400
+    with engine(d, prec, S=S, I=I) as e:
+        e.logl()
+        R = e.get_logl()
+    M32 = d["m0"].astype(np.float32).astype(np.float64)
+    W = orc.slices(M32, d["L"], d["q"][:S], d["coords"], "trilinear")
+    K = d["K"][:I].astype(np.float64)
+    phi0 = orc.fluence(W, K, d["mask"])
+    Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), "true_poisson", phi0=phi0)
+    err = np.abs(R - Ro) / np.maximum(np.abs(Ro), 1.0)
+    log(f"   max rel err {err.max():.2e}  (counts > 256 present: {d['K'][:I].max() > 256})")
+    assert err.max() < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_responsibilities_match_oracle(orc, small, prec):
+    log(f"[resp] prec={prec}")
+    d = small
+    S, I = 1200, 333  # ragged pattern count
+    with engine(d, prec, S=S, I=I) as e:
+        e.logl()
+        tot = e.normalize_resp()
+        P = e.get_resp()
+        lse = e.get_lse()
+        T = e.get_slice_weights()
+    M32 = d["m0"].astype(np.float32).astype(np.float64)
+    W = orc.slices(M32, d["L"], d["q"][:S], d["coords"], "trilinear")
+    K = d["K"][:I].astype(np.float64)
+    phi0 = orc.fluence(W, K, d["mask"])
+    Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), "true_poisson", phi0=phi0)
+    Po, lseo, toto = orc.softmax(Ro)
+    assert np.allclose(P.sum(0), 1.0, atol=1e-4)
+    l1 = np.abs(P - Po).sum(0)
+    log(f"   resp L1 max {l1.max():.2e} mean {l1.mean():.2e}; lse max abs err {np.abs(lse - lseo).max():.2e}")
+    assert l1.max() < 5e-2
+    assert np.abs(lse - lseo).max() < 1e-4 * np.abs(lseo).max()
+    assert abs(tot - toto) < 1e-6 * abs(toto)
+    assert np.allclose(T, P.astype(np.float64).sum(1), rtol=1e-3, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec,insert", [("fp32", "atomic"), ("bf16x2", "atomic"), ("bf16x2", "fused")])
+@pytest.mark.parametrize("interp", ["nearest", "trilinear"])
+def test_maximize_and_compress_match_oracle(orc, small, prec, insert, interp):
+    log(f"[mstep] prec={prec} insert={insert} interp={interp}")
+    d = small
+    S, I = 900, 250
+    rng = np.random.default_rng(0)
+    logits = rng.normal(scale=3.0, size=(S, I))
+    Po = np.exp(logits - logits.max(0))
+    Po /= Po.sum(0)
+    Po[5] = 0.0                # a slice with T_s = 0 is skipped
+    P32 = Po.astype(np.float32)
+    with engine(d, prec, S=S, I=I, insert=insert, interp=interp, blur_sigma=0.5) as e:
+        e.set_resp(P32)
+        e.maximize()
+        _, Wt = e.get_model()
+        e.compress()
+        M, _ = e.get_model()
+    Pin = P32.astype(np.float64)
+    Mnum, Wto, T = orc.mstep(Pin, d["K"][:I].astype(np.float64), d["mask"], d["q"][:S], d["coords"], d["L"], interp)
+    Mo = orc.compress(Mnum, Wto)
+    Mo = orc.blur(Mo, d["L"], 0.5)
+    Mo, _, _ = orc.normalize(Mo)
+    assert np.array_equal(Wt > 0, Wto > 0)
+    assert np.allclose(Wt, Wto, rtol=1e-4, atol=1e-6 * Wto.max())
+    rel = np.linalg.norm(M - Mo) / np.linalg.norm(Mo)
+    log(f"   Wt max rel {np.abs(Wt - Wto).max() / Wto.max():.2e}  model rel L2 {rel:.2e}")
+    tol = 2e-5 if prec == "fp32" else 2e-3  # P enters BF16 hi+lo; the residual below 2^-16 is dropped
+    assert rel < tol
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_full_iterations_match_oracle(orc, small, prec):
+    """Three iterations of the small workload, each started from the oracle's previous model."""
+    log(f"[iterate] prec={prec}")
+    d = small
+    L = d["L"]
+    S, I = d["q"].shape[0], d["K"].shape[0]
+    K = d["K"].astype(np.float64)
+    w64 = d["w"].astype(np.float64)
+    m = d["m0"].astype(np.float32)
+    with engine(d, prec, blur_sigma=0.5) as e:
+        for it in range(3):
+            e.set_model(m)
+            st = e.iterate(it)
+            M, _ = e.get_model()
+            P = e.get_resp()
+            ref = orc.iterate(m.astype(np.float64), L, d["q"], w64, K, d["mask"], d["coords"], "trilinear",
+                              "true_poisson", blur_sigma=0.5, want_P=True)
+            mt = metrics(None, None, P, ref["P"], M, ref["model"])
+            rel_tot = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
+            log(f"   it {it}: {mt} total logL rel {rel_tot:.2e}")
+            assert rel_tot < 1e-6
+            assert mt["model_rel_l2"] < (1e-4 if prec == "fp32" else 2e-3)
+            m = ref["model"].astype(np.float32)
+
+
+def test_chunked_rotations_equal_unchunked(small):
+    log("[chunk]")
+    d = small
+    S, I = 1000, 200
+    outs = []
+    for chunk in (0, 333):
+        with engine(d, "bf16x2", S=S, I=I, slice_chunk=chunk) as e:
+            e.logl()
+            R = e.get_logl()
+            e.normalize_resp()
+            e.maximize()
+            e.compress()
+            outs.append((R, e.get_model()[0]))
+    assert np.allclose(outs[0][0], outs[1][0], rtol=1e-6, atol=1e-3)
+    assert np.allclose(outs[0][1], outs[1][1], rtol=1e-4, atol=1e-6)
+
+
+def test_edge_cases_empty_patterns_and_missing_voxels(orc, small):
+    """All-zero patterns, a model with missing (-1) voxels, nearest interpolation."""
+    log("[edge]")
+    d = dict(small)
+    S, I = 300, 40
+    K = d["K"][:I].copy()
+    K[3] = 0
+    K[17] = 0
+    m = d["m0"].astype(np.float32).copy()
+    m.reshape(32, 32, 32)[:, :, :5] = -1.0
+    d["K"] = K
+    d["m0"] = m
+    with engine(d, "fp32", S=S, I=I, interp="nearest") as e:
+        e.logl()
+        R = e.get_logl()
+        e.normalize_resp()
+        P = e.get_resp()
+    W = orc.slices(m.astype(np.float64), 32, d["q"][:S], d["coords"], "nearest")
+    Kd = K.astype(np.float64)
+    phi0 = orc.fluence(W, Kd, d["mask"])
+    Ro, _ = orc.logl(W, Kd, d["mask"], d["w"][:S].astype(np.float64), "true_poisson", phi0=phi0)
+    Po, _, _ = orc.softmax(Ro)
+    assert np.abs(R - Ro).max() / np.abs(Ro).max() < 2e-5
+    assert np.abs(P - Po).sum(0).max() < 1e-3
+    # an empty pattern's posterior is the rotation prior reweighted by exp(-phi0 sum W_s)
+    assert np.allclose(P[:, 3], Po[:, 3], atol=1e-5)
+
+
+def test_invalid_inputs_are_rejected(small):
+    d = small
+    with pytest.raises(emc_amd.EmcError):
+        emc_amd.Engine(32, 10, 10, precision="bf16x2", diff="poisson")
+    with emc_amd.Engine(32, 60, 10) as e:
+        with pytest.raises(emc_amd.EmcError):
+            e.logl()  # nothing set
+        bad = np.zeros(32 * 32, np.int32)
+        bad[0] = 2
+        with pytest.raises(emc_amd.EmcError):
+            e.set_mask(bad)
+        with pytest.raises(emc_amd.EmcError):
+            e.set_rotations(np.zeros((60, 4), np.float32), np.ones(60, np.float32))  # not unit
