@@ -19,7 +19,7 @@ STATUS_NAMES = {0: "EMC_OK", 1: "EMC_E_INVALID", 2: "EMC_E_HIP", 3: "EMC_E_OOM",
 DIFF = {"absolute": 0, "poisson": 1, "relative": 2, "true_poisson": 3}
 INTERP = {"nearest": 0, "trilinear": 1}
 PRECISION = {"fp32": 0, "bf16x2": 1, "bf16x3": 2}
-INSERT = {"atomic": 0, "fused": 1}
+INSERT = {"atomic": 0, "fused": 1, "binned": 2}
 N_STAGES = 8
 STAGES = ["slice", "logl", "softmax", "mstep", "compress", "total", "logl_gemm", "mstep_gemm"]
 

@@ -134,6 +134,17 @@ emc_status rebuild_active(emc_ctx *c, const float *coords_h, const int32_t *mask
     return EMC_OK;
 }
 
+// Geometry of the binned insertion changed: rebuild once rotations, coords and mask are all set.
+emc_status refresh_bins(emc_ctx *c)
+{
+    c->bins_valid = false;
+    if (c->opt.insert != EMC_INSERT_BINNED || !c->has_rot || !c->has_coords || !c->has_mask) return EMC_OK;
+    std::string err;
+    hipError_t e = build_bins(c, err);
+    if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? EMC_E_OOM : EMC_E_HIP, err);
+    return EMC_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -145,7 +156,7 @@ extern "C" void emc_default_options(emc_options *o)
     o->diff = EMC_DIFF_TRUE_POISSON;
     o->interp = EMC_INTERP_TRILINEAR;
     o->precision = EMC_PREC_BF16X2;
-    o->insert = EMC_INSERT_FUSED;
+    o->insert = EMC_INSERT_BINNED;
     o->sigma_start = 0.4;
     o->sigma_final = 0.12;
     o->sigma_half_life = 25;
@@ -185,6 +196,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
+    dfree(c->bin_vals); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -204,7 +216,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (d->model_side < 4 || d->model_side > 1024 || d->n_rotations < 1 || d->n_patterns < 1 || d->slice_chunk < 0)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad dimensions");
     if (opt.diff < 0 || opt.diff > 3 || opt.interp < 0 || opt.interp > 1 || opt.precision < 0 || opt.precision > 2 ||
-        opt.insert < 0 || opt.insert > 1)
+        opt.insert < 0 || opt.insert > 2)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
     if (opt.precision != EMC_PREC_FP32 && opt.diff != EMC_DIFF_TRUE_POISSON)
         return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: MFMA precision modes need diff = TRUE_POISSON");
@@ -258,7 +270,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK && c->la_terms == 3) A(dalloc(&c->Lm, cp * PpMax), "Lm");
         if (st == EMC_OK) A(dalloc(&c->Ph, Sp * Ip), "Ph");
         if (st == EMC_OK) A(dalloc(&c->Pl, Sp * Ip), "Pl");
-        if (st == EMC_OK) A(dalloc(&c->colsum, PpMax), "colsum");
+        if (st == EMC_OK) A(dalloc(&c->colsum, 2 * (int64_t)PpMax), "colsum");
     } else {
         if (st == EMC_OK) A(dalloc(&c->K32, Ip * PpMax), "K32");
         if (st == EMC_OK) A(dalloc(&c->KT32, Ip * PpMax), "KT32");
@@ -320,7 +332,7 @@ extern "C" emc_status emc_set_rotations(emc_ctx *c, const float *q, const float 
     HIP_TRY(c, e);
     c->has_rot = true;
     c->has_logl = c->has_resp = false;
-    return EMC_OK;
+    return refresh_bins(c);
 }
 
 extern "C" emc_status emc_set_coords(emc_ctx *c, const float *x, const float *y, const float *z)
@@ -341,7 +353,7 @@ extern "C" emc_status emc_set_coords(emc_ctx *c, const float *x, const float *y,
         EMC_TRY(rebuild_active(c, h.data(), m.data()));
     }
     c->has_logl = c->has_resp = false;
-    return EMC_OK;
+    return refresh_bins(c);
 }
 
 extern "C" emc_status emc_set_mask(emc_ctx *c, const int32_t *mask)
@@ -360,7 +372,7 @@ extern "C" emc_status emc_set_mask(emc_ctx *c, const int32_t *mask)
     c->has_mask = true;
     c->has_K = false;  // compaction changed
     c->has_logl = c->has_resp = false;
-    return EMC_OK;
+    return refresh_bins(c);
 }
 
 static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
@@ -692,7 +704,8 @@ static emc_status mstep_enqueue(emc_ctx *c)
         if (c->mfma) HIP_TRY(c, launch_mstep_mfma(c, s0, cs, fused, st));
         else HIP_TRY(c, launch_mstep_f32(c, s0, cs, st));
         gev_rec(c, 1, k, 1);
-        if (!fused) HIP_TRY(c, launch_insert(c, s0, cs, st));
+        if (c->opt.insert == EMC_INSERT_BINNED) HIP_TRY(c, launch_insert_binned(c, s0, cs, st));
+        else if (!fused) HIP_TRY(c, launch_insert(c, s0, cs, st));
     }
     ev_stop(c, EMC_STAGE_MSTEP);
     return EMC_OK;
@@ -702,6 +715,7 @@ extern "C" emc_status emc_maximize(emc_ctx *c)
 {
     if (!c) return fail(c, EMC_E_INVALID, "maximize: null context");
     if (!c->has_resp || !c->has_K || !c->has_rot || !c->has_mask) return fail(c, EMC_E_STATE, "maximize: needs responsibilities");
+    if (c->opt.insert == EMC_INSERT_BINNED && !c->bins_valid) return fail(c, EMC_E_STATE, "maximize: insertion bins not built (needs coords)");
     HIP_TRY(c, hipSetDevice(c->opt.device));
     EMC_TRY(mstep_enqueue(c));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -752,6 +766,7 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
     if (!c) return fail(c, EMC_E_INVALID, "iterate: null context");
     if (!c->has_rot || !c->has_coords || !c->has_mask || !c->has_K || !c->has_model)
         return fail(c, EMC_E_STATE, "iterate: needs rotations, coords, mask, patterns and model");
+    if (c->opt.insert == EMC_INSERT_BINNED && !c->bins_valid) return fail(c, EMC_E_STATE, "iterate: insertion bins not built");
     HIP_TRY(c, hipSetDevice(c->opt.device));
     const double sigma = emc_sigma_schedule(iteration, c->opt.sigma_start, c->opt.sigma_final, c->opt.sigma_half_life);
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL], c->stream);

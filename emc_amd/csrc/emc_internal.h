@@ -78,7 +78,7 @@ struct emc_ctx {
     // slicing stage
     double *sumW = nullptr;   // [Sp]
     int32_t *ns = nullptr;    // [Sp]
-    double *colsum = nullptr; // [Pp] sum over rotations of log W (centering)
+    double *colsum = nullptr; // [Pp][2] per pixel: sum over rotations of log W, valid count
     float *colmean = nullptr; // [Pp]
     float *bias = nullptr;    // [Sp] per-rotation additive term of the logits
     float *rscale = nullptr;  // [Sp] chi^2 factor 1/(2 n_s sigma^2)
@@ -95,6 +95,11 @@ struct emc_ctx {
     float *U = nullptr;                                // [chunkp][Pp] (atomic insertion)
     double *scal = nullptr;                            // [SC_N] device scalars
     double *scal_h = nullptr;                          // pinned host mirror
+    // binned insertion: (s, j) pair indices sorted by (chunk, model brick); work units
+    uint32_t *bin_vals = nullptr;                      // [S * Pa]
+    int32_t *bin_units = nullptr;                      // [n_units][3] = key, begin, end
+    std::vector<int> bin_chunk_units;                  // unit range of each rotation chunk
+    bool bins_valid = false;
 };
 
 namespace emc {
@@ -109,7 +114,7 @@ hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st);
 hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st);
 // Fluence, per-rotation bias / chi^2 scale, column means.
 hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st);
-// Per-pattern constant: cst[i] = sum_j K[i,j] colmean[j] (MFMA) + sumK[i] log phi0.
+// Per-pattern constant: cst[i] = sum_j K[i,j] colmean[j] (MFMA path) - B0.
 hipError_t launch_pattern_const(const emc_ctx *c, bool centred, hipStream_t st);
 // FP32 slices of rotations [s0, s0+cs) into A0/A1 (rows up to chunkp zero-padded).
 hipError_t launch_slice_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
@@ -129,6 +134,9 @@ hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipSt
 hipError_t launch_softmax(const emc_ctx *c, hipStream_t st);
 // Atomic insertion of U rows [s0, s0+cs).
 hipError_t launch_insert(const emc_ctx *c, int s0, int cs, hipStream_t st);
+// Brick-binned insertion: build the sorted pair lists (setup; allocates), then per chunk.
+hipError_t build_bins(emc_ctx *c, std::string &err);
+hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st);
 // Compress: divide, blur, normalise.
 hipError_t launch_divide(const emc_ctx *c, hipStream_t st);
 hipError_t launch_blur(const emc_ctx *c, hipStream_t st);

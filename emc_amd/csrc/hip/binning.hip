@@ -1,0 +1,249 @@
+// Maximize, scatter half, binned: insertion without global-atomic contention.
+//
+// The geometry of insertion (which voxels a rotated pixel touches, with which trilinear weights)
+// depends only on the rotation list, the detector coordinates and the mask, not on the
+// iteration.  So once per geometry the valid (rotation s, active pixel j) pairs are sorted by the
+// model brick (kBrick^3 voxels) that holds their base voxel, per rotation chunk, with a stable
+// radix sort (rocPRIM).  Every iteration then runs one workgroup per slice of a brick's list:
+// it accumulates U[s, j] and T_s with LDS atomics into a (kBrick+1)^3 private copy of the brick
+// (the +1 is the trilinear halo) and flushes it to the model numerator and weight once.  Reads
+// of the item list, of U and of the pixel coordinates are contiguous because the list is
+// ordered by (s, j) within each brick.  This replaces the 16 global atomics per (s, j) of the
+// reference's insert_slices_kernel (emc_atomic.cu:33-85, 183-200) by 16 LDS atomics.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "../emc_internal.h"
+#include "geom.h"
+
+namespace emc {
+namespace {
+
+constexpr int kBrick = 8;
+constexpr int kHalo = kBrick + 1;
+constexpr int kUnit = 4096;  // items per workgroup
+
+struct BinGeom {
+    int L, nb, S, Pa, Pp, chunk, interp;
+};
+
+__device__ __forceinline__ bool item_cell(const float *__restrict__ rotm, const float *__restrict__ axyz,
+                                          const BinGeom &g, int s, int j, int &bx, int &by, int &bz, float &fx,
+                                          float &fy, float &fz)
+{
+    float m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = rotm[12 * (size_t)s + k];
+    float ux, uy, uz;
+    rot_point(m, axyz[j], axyz[g.Pp + j], axyz[2 * g.Pp + j], model_c(g.L), ux, uy, uz);
+    if (g.interp == EMC_INTERP_NEAREST) {
+        bx = nearest(ux);
+        by = nearest(uy);
+        bz = nearest(uz);
+        fx = fy = fz = 0.0f;
+        return bx >= 0 && bx < g.L && by >= 0 && by < g.L && bz >= 0 && bz < g.L;  // emc_atomic.cu:67-69
+    }
+    return tri_axis(ux, g.L, bx, fx) && tri_axis(uy, g.L, by, fy) && tri_axis(uz, g.L, bz, fz);
+}
+
+__global__ void k_bin_keys(const float *__restrict__ rotm, const float *__restrict__ axyz, BinGeom g,
+                           int64_t n, uint32_t invalid_key, uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(idx / g.Pa), j = (int)(idx % g.Pa);
+        int bx, by, bz;
+        float fx, fy, fz;
+        uint32_t key = invalid_key;
+        if (item_cell(rotm, axyz, g, s, j, bx, by, bz, fx, fy, fz)) {
+            const int brick = ((bz / kBrick) * g.nb + by / kBrick) * g.nb + bx / kBrick;
+            key = (uint32_t)((s / g.chunk) * g.nb * g.nb * g.nb + brick);
+        }
+        keys[idx] = key;
+        vals[idx] = (uint32_t)idx;
+    }
+}
+
+__global__ void k_bin_count(const uint32_t *__restrict__ keys, int64_t n, uint32_t nkeys, int32_t *__restrict__ counts)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = keys[t];
+        if (k < nkeys && (t == 0 || keys[t - 1] != k)) counts[2 * k] = (int32_t)t;       // first position
+        if (k < nkeys && (t == n - 1 || keys[t + 1] != k)) counts[2 * k + 1] = (int32_t)(t + 1);  // end
+    }
+}
+
+template <int INTERP>
+__global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict__ units, const uint32_t *__restrict__ vals,
+                                                       const float *__restrict__ U, int s0, const double *__restrict__ T,
+                                                       const float *__restrict__ rotm, const float *__restrict__ axyz,
+                                                       BinGeom g, float *__restrict__ Mnum, float *__restrict__ Wt)
+{
+    __shared__ float accM[kHalo * kHalo * kHalo], accW[kHalo * kHalo * kHalo];
+    const int u = blockIdx.x;
+    const int brick = units[3 * u] % (g.nb * g.nb * g.nb);
+    const int begin = units[3 * u + 1], end = units[3 * u + 2];
+    const int ox = (brick % g.nb) * kBrick, oy = ((brick / g.nb) % g.nb) * kBrick, oz = (brick / (g.nb * g.nb)) * kBrick;
+    for (int k = threadIdx.x; k < kHalo * kHalo * kHalo; k += 256) {
+        accM[k] = 0.0f;
+        accW[k] = 0.0f;
+    }
+    __syncthreads();
+    for (int t = begin + threadIdx.x; t < end; t += 256) {
+        const uint32_t idx = vals[t];
+        const int s = (int)(idx / (uint32_t)g.Pa), j = (int)(idx % (uint32_t)g.Pa);
+        const double Td = T[s];
+        if (!(Td > 1e-10)) continue;  // emc_atomic.cu:190
+        const float val = U[(size_t)(s - s0) * g.Pp + j];
+        if (!(val >= 0.0f)) continue;  // emc_atomic.cu:56
+        const float Ts = (float)Td;
+        int bx, by, bz;
+        float fx, fy, fz;
+        item_cell(rotm, axyz, g, s, j, bx, by, bz, fx, fy, fz);
+        const int lx = bx - ox, ly = by - oy, lz = bz - oz;
+        if (INTERP == EMC_INTERP_NEAREST) {
+            const int l = (lz * kHalo + ly) * kHalo + lx;
+            atomicAdd(accM + l, val);
+            atomicAdd(accW + l, Ts);
+        } else {
+            const float gw[2][3] = {{1.0f - fx, 1.0f - fy, 1.0f - fz}, {fx, fy, fz}};
+#pragma unroll
+            for (int dz = 0; dz < 2; ++dz)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int dx = 0; dx < 2; ++dx) {
+                        const float w = gw[dx][0] * gw[dy][1] * gw[dz][2];
+                        const int l = ((lz + dz) * kHalo + ly + dy) * kHalo + lx + dx;
+                        atomicAdd(accM + l, w * val);
+                        atomicAdd(accW + l, w * Ts);
+                    }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kHalo * kHalo * kHalo; k += 256) {
+        const float a = accM[k], b = accW[k];
+        if (a == 0.0f && b == 0.0f) continue;
+        const int x = ox + k % kHalo, y = oy + (k / kHalo) % kHalo, z = oz + k / (kHalo * kHalo);
+        if (x >= g.L || y >= g.L || z >= g.L) continue;
+        const size_t v = ((size_t)z * g.L + y) * g.L + x;
+        atomicAdd(Mnum + v, a);
+        atomicAdd(Wt + v, b);
+    }
+}
+
+BinGeom geom_of(const emc_ctx *c)
+{
+    BinGeom g;
+    g.L = c->L;
+    g.nb = (c->L + kBrick - 1) / kBrick;
+    g.S = c->S;
+    g.Pa = c->Pa;
+    g.Pp = c->Pp;
+    g.chunk = c->chunk;
+    g.interp = c->opt.interp;
+    return g;
+}
+
+}  // namespace
+
+// Builds c->bin_vals / c->bin_units / c->bin_chunk_units.  Allocates (setup time only).
+hipError_t build_bins(emc_ctx *c, std::string &err)
+{
+    hipStream_t st = c->stream;
+    const BinGeom g = geom_of(c);
+    const int64_t n = (int64_t)c->S * c->Pa;
+    if (n >= ((int64_t)1 << 31)) {
+        err = "binned insertion supports S * active pixels < 2^31";
+        return hipErrorInvalidValue;
+    }
+    const int nch = (c->S + c->chunk - 1) / c->chunk;
+    const uint32_t nbr = (uint32_t)g.nb * g.nb * g.nb;
+    const uint32_t nkeys = (uint32_t)nch * nbr;
+    int end_bit = 1;
+    while (end_bit < 32 && ((uint64_t)1 << end_bit) <= nkeys) ++end_bit;
+    uint32_t *kin = nullptr, *vin = nullptr, *kout = nullptr;
+    void *tmp = nullptr;
+    int32_t *cnt = nullptr;
+    size_t tmp_bytes = 0;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    if (c->bin_vals) (void)hipFree(c->bin_vals);
+    if (c->bin_units) (void)hipFree(c->bin_units);
+    c->bin_vals = nullptr;
+    c->bin_units = nullptr;
+    chk(hipMalloc((void **)&kin, sizeof(uint32_t) * n));
+    chk(hipMalloc((void **)&vin, sizeof(uint32_t) * n));
+    chk(hipMalloc((void **)&kout, sizeof(uint32_t) * n));
+    chk(hipMalloc((void **)&c->bin_vals, sizeof(uint32_t) * n));
+    chk(hipMalloc((void **)&cnt, sizeof(int32_t) * 2 * (size_t)nkeys));
+    if (e == hipSuccess) {
+        const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_bin_keys, dim3(blocks), dim3(256), 0, st, c->rotm, c->axyz, g, n, nkeys, kin, vin);
+        chk(hipGetLastError());
+    }
+    if (e == hipSuccess)
+        chk(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kin, kout, vin, c->bin_vals, (size_t)n, 0, end_bit, st));
+    if (e == hipSuccess) chk(hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16)));
+    if (e == hipSuccess)
+        chk(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, c->bin_vals, (size_t)n, 0, end_bit, st));
+    if (e == hipSuccess) chk(hipMemsetAsync(cnt, 0, sizeof(int32_t) * 2 * (size_t)nkeys, st));
+    if (e == hipSuccess) {
+        const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_bin_count, dim3(blocks), dim3(256), 0, st, kout, n, nkeys, cnt);
+        chk(hipGetLastError());
+    }
+    std::vector<int32_t> h(2 * (size_t)nkeys, 0);
+    if (e == hipSuccess) chk(hipMemcpyAsync(h.data(), cnt, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost, st));
+    if (e == hipSuccess) chk(hipStreamSynchronize(st));
+    // work units (key, begin, end), at most kUnit items each, ordered by key = chunk-major
+    std::vector<int32_t> units;
+    c->bin_chunk_units.assign(nch + 1, 0);
+    for (uint32_t k = 0; k < nkeys; ++k) {
+        const int ch = (int)(k / nbr);
+        const int32_t b = h[2 * k], en = h[2 * k + 1];
+        for (int32_t t = b; t < en; t += kUnit) {
+            units.push_back((int32_t)k);
+            units.push_back(t);
+            units.push_back(std::min(en, t + kUnit));
+        }
+        c->bin_chunk_units[ch + 1] = (int)(units.size() / 3);
+    }
+    for (int ch = 1; ch <= nch; ++ch) c->bin_chunk_units[ch] = std::max(c->bin_chunk_units[ch], c->bin_chunk_units[ch - 1]);
+    if (e == hipSuccess && !units.empty()) {
+        chk(hipMalloc((void **)&c->bin_units, sizeof(int32_t) * units.size()));
+        if (e == hipSuccess)
+            chk(hipMemcpy(c->bin_units, units.data(), sizeof(int32_t) * units.size(), hipMemcpyHostToDevice));
+    }
+    (void)hipFree(kin);
+    (void)hipFree(vin);
+    (void)hipFree(kout);
+    (void)hipFree(tmp);
+    (void)hipFree(cnt);
+    if (e != hipSuccess && err.empty()) err = std::string("build_bins: ") + hipGetErrorString(e);
+    c->bins_valid = e == hipSuccess;
+    return e;
+}
+
+hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st)
+{
+    (void)cs;
+    const int k = s0 / c->chunk;
+    const int u0 = c->bin_chunk_units[k], u1 = c->bin_chunk_units[k + 1];
+    if (u1 <= u0) return hipSuccess;
+    const BinGeom g = geom_of(c);
+    if (c->opt.interp == EMC_INTERP_NEAREST)
+        hipLaunchKernelGGL(k_insert_binned<EMC_INTERP_NEAREST>, dim3(u1 - u0), dim3(256), 0, st, c->bin_units + 3 * u0,
+                           c->bin_vals, c->U, s0, c->T, c->rotm, c->axyz, g, c->Mnum, c->Wt);
+    else
+        hipLaunchKernelGGL(k_insert_binned<EMC_INTERP_TRILINEAR>, dim3(u1 - u0), dim3(256), 0, st,
+                           c->bin_units + 3 * u0, c->bin_vals, c->U, s0, c->T, c->rotm, c->axyz, g, c->Mnum, c->Wt);
+    return hipGetLastError();
+}
+
+}  // namespace emc

@@ -88,7 +88,8 @@ __global__ void k_fill(T *__restrict__ p, T v, int64_t n)
         p[i] = v;
 }
 
-// Per-rotation slice sum over V_s = {active j : W > 0}, |V_s|, and per-pixel sums of log W.
+// Per-rotation slice sum over V_s = {active j : W > 0}, |V_s|, and per pixel the sum of log W and
+// the number of rotations with a valid sample (colsum[2j], colsum[2j+1]).
 // kStatsG rotations per workgroup; each thread walks pixels j = tid, tid+256, ... and keeps the
 // kStatsG partial sums in registers.
 template <int INTERP>
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
     for (int j = tid; j < Pa; j += 256) {
         const float X = axyz[j], Y = axyz[Pp + j], Z = axyz[2 * Pp + j];
         float col = 0.0f;
+        int colv = 0;
 #pragma unroll
         for (int g = 0; g < kStatsG; ++g) {
             if (g < ng) {
@@ -124,10 +126,14 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
                     sw[g] += W;
                     cnt[g] += 1;
                     col += logf(W);
+                    colv += 1;
                 }
             }
         }
-        if (colsum) atomicAdd(colsum + j, (double)col);
+        if (colsum) {
+            atomicAdd(colsum + 2 * j, (double)col);
+            atomicAdd(colsum + 2 * j + 1, (double)colv);
+        }
     }
     const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
@@ -143,17 +149,20 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
     }
 }
 
-// FP32 slice rows for the plain contraction.  KIND: TRUE_POISSON -> A0 = log W (0 if invalid);
+// FP32 slice rows for the plain contraction.  KIND: TRUE_POISSON -> A0 = log(phi0 W) (0 if
+// invalid; log phi0 is read from the device scalars written by k_bias);
 // chi^2 kinds -> A0 = W, A1 = pixel weight a (0 if invalid):
 //   poisson (reference chi^2) a = 1/(W+0.02), absolute a = 1, relative a = 1/W^2.
 template <int INTERP, int KIND>
 __global__ __launch_bounds__(256) void k_slice_f32(const float *__restrict__ model, int L,
                                                    const float *__restrict__ rotm, int s0, int cs,
                                                    const float *__restrict__ axyz, int Pa, int Pp,
+                                                   const double *__restrict__ scal,
                                                    float *__restrict__ A0, float *__restrict__ A1)
 {
     const int r = blockIdx.x;
     const bool row_ok = r < cs;
+    const float logphi0 = KIND == EMC_DIFF_TRUE_POISSON ? (float)scal[SC_LOGPHI0] : 0.0f;
     float m[9];
     if (row_ok) load_rot(rotm, s0 + r, m);
     for (int j = threadIdx.x; j < Pp; j += 256) {
@@ -162,7 +171,7 @@ __global__ __launch_bounds__(256) void k_slice_f32(const float *__restrict__ mod
             const float W = sample<INTERP>(model, L, m, axyz[j], axyz[Pp + j], axyz[2 * Pp + j]);
             if (W > 0.0f) {
                 if (KIND == EMC_DIFF_TRUE_POISSON) {
-                    a0 = logf(W);
+                    a0 = logf(W) + logphi0;
                 } else {
                     a0 = W;
                     a1 = KIND == EMC_DIFF_POISSON ? 1.0f / (W + 0.02f)
@@ -175,25 +184,27 @@ __global__ __launch_bounds__(256) void k_slice_f32(const float *__restrict__ mod
     }
 }
 
-// Centred log-slice x = (W > 0 ? log W : 0) - colmean[j] split into NT BF16 terms
+// Centred log-slice x = (W > 0 ? log(phi0 W) : 0) - colmean[j] split into NT BF16 terms
 // (hi, [mid,] lo) whose sum carries ~8*NT significant bits of x.
 template <int INTERP, int NT>
 __global__ __launch_bounds__(256) void k_slice_bf16(const float *__restrict__ model, int L,
                                                     const float *__restrict__ rotm, int s0, int cs,
                                                     const float *__restrict__ axyz, int Pa, int Pp,
                                                     const float *__restrict__ colmean,
+                                                    const double *__restrict__ scal,
                                                     bf16_t *__restrict__ Lh, bf16_t *__restrict__ Lm,
                                                     bf16_t *__restrict__ Ll)
 {
     const int r = blockIdx.x;
     const bool row_ok = r < cs;
+    const float logphi0 = (float)scal[SC_LOGPHI0];
     float m[9];
     if (row_ok) load_rot(rotm, s0 + r, m);
     for (int j = threadIdx.x; j < Pp; j += 256) {
         float x = 0.0f;
         if (row_ok && j < Pa) {
             const float W = sample<INTERP>(model, L, m, axyz[j], axyz[Pp + j], axyz[2 * Pp + j]);
-            x = (W > 0.0f ? logf(W) : 0.0f) - colmean[j];
+            x = (W > 0.0f ? logf(W) + logphi0 : 0.0f) - colmean[j];
         }
         const bf16_t h = f2bf(x);
         const float r1 = x - bf2f(h);
@@ -267,12 +278,14 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
         bias[s] = b;
         rscale[s] = rs;
     }
+    // column mean of log(phi0 W) over rotations (invalid samples count as 0)
+    const double lp = log(phi0);
     for (int j = tid; j < Pp; j += 1024)
-        colmean[j] = (colsum && j < Pa && S > 0) ? (float)(colsum[j] / S) : 0.0f;
+        colmean[j] = (colsum && j < Pa && S > 0) ? (float)((colsum[2 * j] + colsum[2 * j + 1] * lp) / S) : 0.0f;
     if (tid == 0) scal[SC_B0] = B0;
 }
 
-// cst_i = sum_j K_ij colmean_j + sumK_i log(phi0) - B0   (FP64; one workgroup per pattern)
+// cst_i = sum_j K_ij colmean_j - B0   (FP64; one workgroup per pattern)
 __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__ K32,
                                                        const bf16_t *__restrict__ K16,
                                                        const bf16_t *__restrict__ K16lo, int I, int Pp,
@@ -299,7 +312,7 @@ __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
     __syncthreads();
     if (threadIdx.x == 0)
-        cst[i] = red[0] + red[1] + red[2] + red[3] + sumK[i] * scal[SC_LOGPHI0] - scal[SC_B0];
+        cst[i] = red[0] + red[1] + red[2] + red[3] - scal[SC_B0];
 }
 
 }  // namespace
@@ -330,7 +343,7 @@ hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st)
 hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st)
 {
     if (want_colsum) {
-        hipError_t e = launch_fill_f64(c->colsum, 0.0, c->Pp, st);
+        hipError_t e = launch_fill_f64(c->colsum, 0.0, 2 * (int64_t)c->Pp, st);
         if (e != hipSuccess) return e;
     }
     const dim3 grid((c->S + kStatsG - 1) / kStatsG);
@@ -366,19 +379,19 @@ static hipError_t slice_f32_interp(const emc_ctx *c, int s0, int cs, hipStream_t
     switch (c->opt.diff) {
     case EMC_DIFF_TRUE_POISSON:
         hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_TRUE_POISSON>), grid, blk, 0, st, c->model, c->L,
-                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->scal, c->A0, c->A1);
         break;
     case EMC_DIFF_POISSON:
         hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_POISSON>), grid, blk, 0, st, c->model, c->L,
-                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->scal, c->A0, c->A1);
         break;
     case EMC_DIFF_ABSOLUTE:
         hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_ABSOLUTE>), grid, blk, 0, st, c->model, c->L,
-                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->scal, c->A0, c->A1);
         break;
     default:
         hipLaunchKernelGGL((k_slice_f32<INTERP, EMC_DIFF_RELATIVE>), grid, blk, 0, st, c->model, c->L,
-                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->A0, c->A1);
+                           c->rotm, s0, cs, c->axyz, c->Pa, c->Pp, c->scal, c->A0, c->A1);
         break;
     }
     return hipGetLastError();
@@ -397,17 +410,17 @@ hipError_t launch_slice_bf16(const emc_ctx *c, int s0, int cs, hipStream_t st)
     if (c->la_terms == 3) {
         if (nn)
             hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_NEAREST, 3>), grid, blk, 0, st, c->model, c->L, c->rotm,
-                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->scal, c->Lh, c->Lm, c->Ll);
         else
             hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_TRILINEAR, 3>), grid, blk, 0, st, c->model, c->L, c->rotm,
-                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->scal, c->Lh, c->Lm, c->Ll);
     } else {
         if (nn)
             hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_NEAREST, 2>), grid, blk, 0, st, c->model, c->L, c->rotm,
-                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->scal, c->Lh, c->Lm, c->Ll);
         else
             hipLaunchKernelGGL((k_slice_bf16<EMC_INTERP_TRILINEAR, 2>), grid, blk, 0, st, c->model, c->L, c->rotm,
-                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->Lh, c->Lm, c->Ll);
+                               s0, cs, c->axyz, c->Pa, c->Pp, c->colmean, c->scal, c->Lh, c->Lm, c->Ll);
     }
     return hipGetLastError();
 }

@@ -54,8 +54,10 @@ enum { EMC_PREC_FP32 = 0, EMC_PREC_BF16X2 = 1, EMC_PREC_BF16X3 = 2 };
 /* How the M-step inserts the weighted slice sums into the model.
  *   ATOMIC : U = P^T.K is materialised per rotation chunk, then scattered with float atomics
  *            (emc_atomic.cu:33-85, 183-200).
- *   FUSED  : the MFMA M-step scatters from its accumulators (no U buffer). */
-enum { EMC_INSERT_ATOMIC = 0, EMC_INSERT_FUSED = 1 };
+ *   FUSED  : the MFMA M-step scatters from its accumulators with float atomics (no U buffer).
+ *   BINNED : U is materialised; (rotation, pixel) pairs pre-sorted by model brick are
+ *            accumulated per brick in LDS and flushed once (no contended global atomics). */
+enum { EMC_INSERT_ATOMIC = 0, EMC_INSERT_FUSED = 1, EMC_INSERT_BINNED = 2 };
 
 typedef struct emc_dims {
     int32_t model_side;  /* L: model L^3, patterns L x L (configs.h:14 model_side) */
@@ -106,7 +108,7 @@ typedef struct emc_ctx emc_ctx;
 
 /* ---- context ----------------------------------------------------------------------------- */
 
-/* Defaults: TRUE_POISSON, TRILINEAR, BF16X2, FUSED, sigma 0.4 -> 0.12 over 25 (emc.conf:8-10). */
+/* Defaults: TRUE_POISSON, TRILINEAR, BF16X2, BINNED, sigma 0.4 -> 0.12 over 25 (emc.conf:8-10). */
 void emc_default_options(emc_options *opt);
 
 /* Replaces the allocation block of main.cpp:158-209 and cuda_set_device (emc_cuda.cu:866-869).

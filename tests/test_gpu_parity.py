@@ -120,7 +120,8 @@ def test_responsibilities_match_oracle(orc, small, prec):
     assert np.allclose(T, P.astype(np.float64).sum(1), rtol=1e-3, atol=1e-6)
 
 
-@pytest.mark.parametrize("prec,insert", [("fp32", "atomic"), ("bf16x2", "atomic"), ("bf16x2", "fused")])
+@pytest.mark.parametrize("prec,insert", [("fp32", "atomic"), ("fp32", "binned"), ("bf16x2", "atomic"),
+                                         ("bf16x2", "fused"), ("bf16x2", "binned")])
 @pytest.mark.parametrize("interp", ["nearest", "trilinear"])
 def test_maximize_and_compress_match_oracle(orc, small, prec, insert, interp):
     log(f"[mstep] prec={prec} insert={insert} interp={interp}")
