@@ -40,7 +40,7 @@ def traffic_from_profile(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
-def run_engine(d, prec, steps, warmup, insert="fused"):
+def run_engine(d, prec, steps, warmup, insert="binned"):
     import emc_amd
     L = d["L"]
     S, I = d["q"].shape[0], d["K"].shape[0]
@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
-    ap.add_argument("--insert", default="fused", choices=["fused", "atomic"])
+    ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

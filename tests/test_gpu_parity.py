@@ -14,10 +14,17 @@ def log(msg):
     print(msg, flush=True)
 
 
+def rel_logl(R_gpu, R_orc):
+    """max over (s, i) of |dR| / max_s |R[s, i]|: error relative to the pattern's logL magnitude
+    (a single logit can cross zero, so |dR| / |R| per entry is ill-conditioned)."""
+    scale = np.maximum(np.abs(R_orc).max(0, keepdims=True), 1.0)
+    return float(np.max(np.abs(R_gpu.astype(np.float64) - R_orc) / scale))
+
+
 def metrics(R_gpu, R_orc, P_gpu, P_orc, M_gpu, M_orc):
     out = {}
     if R_gpu is not None:
-        out["rel_logl"] = float(np.max(np.abs(R_gpu.astype(np.float64) - R_orc) / np.maximum(np.abs(R_orc), 1.0)))
+        out["rel_logl"] = rel_logl(R_gpu, R_orc)
     if P_gpu is not None:
         l1 = np.abs(P_gpu.astype(np.float64) - P_orc).sum(0)
         out["resp_l1_max"] = float(l1.max())
@@ -71,9 +78,9 @@ def test_logl_fp32_matches_oracle(orc, small, diff, interp):
     K = d["K"][:I].astype(np.float64)
     phi0 = orc.fluence(W, K, d["mask"]) if diff == "true_poisson" else 1.0
     Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), diff, sigma=sigma, phi0=phi0)
-    err = np.abs(R - Ro) / np.maximum(np.abs(Ro), 1.0)
-    log(f"   max rel err {err.max():.2e}")
-    assert err.max() < 2e-5
+    err = rel_logl(R, Ro)
+    log(f"   rel logL err {err:.2e}, max abs err {np.abs(R - Ro).max():.2e}")
+    assert err < 1e-5
 
 
 @pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
@@ -89,9 +96,10 @@ def test_logl_mfma_matches_oracle(orc, small, prec):
     K = d["K"][:I].astype(np.float64)
     phi0 = orc.fluence(W, K, d["mask"])
     Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), "true_poisson", phi0=phi0)
-    err = np.abs(R - Ro) / np.maximum(np.abs(Ro), 1.0)
-    log(f"   max rel err {err.max():.2e}  (counts > 256 present: {d['K'][:I].max() > 256})")
-    assert err.max() < 1e-4
+    err = rel_logl(R, Ro)
+    log(f"   rel logL err {err:.2e}, max abs err {np.abs(R - Ro).max():.2e} "
+        f"(counts > 256 present: {d['K'][:I].max() > 256})")
+    assert err < (1e-4 if prec == "bf16x2" else 1e-5)
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
@@ -114,9 +122,9 @@ def test_responsibilities_match_oracle(orc, small, prec):
     assert np.allclose(P.sum(0), 1.0, atol=1e-4)
     l1 = np.abs(P - Po).sum(0)
     log(f"   resp L1 max {l1.max():.2e} mean {l1.mean():.2e}; lse max abs err {np.abs(lse - lseo).max():.2e}")
-    assert l1.max() < 5e-2
-    assert np.abs(lse - lseo).max() < 1e-4 * np.abs(lseo).max()
-    assert abs(tot - toto) < 1e-6 * abs(toto)
+    assert l1.max() < 5e-3
+    assert np.abs(lse - lseo).max() < 1e-5 * np.abs(lseo).max()
+    assert abs(tot - toto) < 1e-5 * abs(toto)
     assert np.allclose(T, P.astype(np.float64).sum(1), rtol=1e-3, atol=1e-6)
 
 
@@ -173,8 +181,9 @@ def test_full_iterations_match_oracle(orc, small, prec):
             mt = metrics(None, None, P, ref["P"], M, ref["model"])
             rel_tot = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
             log(f"   it {it}: {mt} total logL rel {rel_tot:.2e}")
-            assert rel_tot < 1e-6
-            assert mt["model_rel_l2"] < (1e-4 if prec == "fp32" else 2e-3)
+            assert rel_tot < 1e-5
+            assert mt["resp_l1_max"] < 5e-3
+            assert mt["model_rel_l2"] < 1e-4
             m = ref["model"].astype(np.float32)
 
 
@@ -217,7 +226,7 @@ def test_edge_cases_empty_patterns_and_missing_voxels(orc, small):
     phi0 = orc.fluence(W, Kd, d["mask"])
     Ro, _ = orc.logl(W, Kd, d["mask"], d["w"][:S].astype(np.float64), "true_poisson", phi0=phi0)
     Po, _, _ = orc.softmax(Ro)
-    assert np.abs(R - Ro).max() / np.abs(Ro).max() < 2e-5
+    assert rel_logl(R, Ro) < 1e-5
     assert np.abs(P - Po).sum(0).max() < 1e-3
     # an empty pattern's posterior is the rotation prior reweighted by exp(-phi0 sum W_s)
     assert np.allclose(P[:, 3], Po[:, 3], atol=1e-5)
