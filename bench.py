@@ -40,12 +40,13 @@ def traffic_from_profile(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
-def run_engine(d, prec, steps, warmup, insert="binned"):
+def run_engine(d, prec, steps, warmup, insert="binned", mfma_tile=0):
     import emc_amd
     L = d["L"]
     S, I = d["q"].shape[0], d["K"].shape[0]
     t = time.perf_counter()
-    e = emc_amd.Engine(L, S, I, precision=prec, insert=insert, blur_sigma=d["cfg"]["blur"], timing=1)
+    e = emc_amd.Engine(L, S, I, precision=prec, insert=insert, blur_sigma=d["cfg"]["blur"], timing=1,
+                       mfma_tile=mfma_tile)
     e.setup(d["q"], d["w"], d["coords"], d["mask"], d["K"], d["model0"])
     e.normalize_model()
     log(f"[bench] {prec}: setup {time.perf_counter() - t:.1f}s")
@@ -95,6 +96,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
+    ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -119,6 +121,16 @@ def main():
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
         fp32 = {"value": 1000.0 / ms32, "ms_per_step": ms32, "stage_ms": st32,
                 "logl_gemm_tflops": 2.0 * I * S * Pa / (st32["logl_gemm"] * 1e-3) / 1e12}
+
+    variants = {}
+    if args.variants:
+        for name, kw in {"bf16x2_tile128": dict(prec="bf16x2", mfma_tile=1),
+                         "bf16x2_fused_insert": dict(prec="bf16x2", insert="fused"),
+                         "bf16x2_atomic_insert": dict(prec="bf16x2", insert="atomic"),
+                         "bf16x3": dict(prec="bf16x3")}.items():
+            prec = kw.pop("prec")
+            ms_v, st_v, _ = run_engine(d, prec, 2, 1, **kw)
+            variants[name] = {"ms_per_step": ms_v, "stage_ms": st_v}
 
     flops = 2.0 * I * S * Pa  # one logical K.logW^T product, counted once (the BF16 split is overhead)
     t_logl = st_mfma["logl_gemm"] * 1e-3
@@ -161,6 +173,7 @@ def main():
         "total_logl": last["total_logl"],
         "roofline": roof,
         "cpu_baseline": cpu,
+        "variants": variants or None,
     }
     print(json.dumps(out))
 

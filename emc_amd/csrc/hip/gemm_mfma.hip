@@ -219,6 +219,139 @@ hipError_t run(dim3 grid, hipStream_t st, const bf16_t *A0, const bf16_t *A1, co
     return hipGetLastError();
 }
 
+// ---- large-tile variant ---------------------------------------------------------------------
+// 256 x BN2 macro tile, 512 threads = 8 wavefronts (4 along M x 2 along N), each wave a 64 x BN2/2
+// block of 2 x (BN2/64) 32x32 tiles, so every A fragment (NA terms) feeds BN2/64 MFMAs and every
+// B fragment NA * 2.  BK = 32; two LDS stages (rows padded to 40 BF16 = 80 B: the 16-byte
+// fragment reads of 16 lanes cover all 64 banks once), one barrier per k-step: the global loads
+// of step k+1 are in flight while step k is multiplied, then land in the other stage.
+// Workgroups are renumbered so that each XCD (round-robin dispatch: workgroup b runs on XCD b % 8)
+// works on a contiguous band of tiles, which share A row panels in that XCD's L2.
+constexpr int BM2 = 256, BK2 = 32, LDK2 = BK2 + 8;
+
+template <int NA, int EPI, int BN2>
+__global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                                    const bf16_t *__restrict__ A2, int64_t lda,
+                                                    const bf16_t *__restrict__ B0, int64_t ldb, int Kd, int ntn,
+                                                    EpiArgs e)
+{
+    constexpr int TN = BN2 / 64;                       // 32-wide tiles per wave along N
+    constexpr int BCH = BN2 * (BK2 / 8) / 512;         // 16-byte B chunks per thread per stage
+    constexpr int STAGE = (NA * BM2 + BN2) * LDK2;     // BF16 elements per stage
+    __shared__ __attribute__((aligned(16))) bf16_t smem[2 * STAGE];
+
+    // XCD-aware renumbering (bijective for any grid size)
+    const int nwg = gridDim.x, b = blockIdx.x;
+    const int base = nwg / 8, rem = nwg % 8, xcd = b % 8, loc = b / 8;
+    const int t = xcd * base + min(xcd, rem) + loc;
+    const int m0 = (t / ntn) * BM2, n0 = (t % ntn) * BN2;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN2 / 2);
+    const int lrow = tid >> 2, lcol = (tid & 3) * 8;  // one 16-byte chunk: row, k offset
+
+    const bf16_t *ga[NA];
+    ga[0] = A0 + (size_t)(m0 + lrow) * lda + lcol;
+    if (NA > 1) ga[1] = A1 + (size_t)(m0 + lrow) * lda + lcol;
+    if (NA > 2) ga[NA - 1] = A2 + (size_t)(m0 + lrow) * lda + lcol;
+    const bf16_t *gb = B0 + (size_t)(n0 + lrow) * ldb + lcol;
+
+    u32x4 ra[NA][2], rb[BCH];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) ra[a][h] = *reinterpret_cast<const u32x4 *>(ga[a] + (size_t)128 * h * lda + k0);
+#pragma unroll
+        for (int h = 0; h < BCH; ++h) rb[h] = *reinterpret_cast<const u32x4 *>(gb + (size_t)128 * h * ldb + k0);
+    };
+    auto sstore = [&](int stage) {
+        bf16_t *S = smem + stage * STAGE;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<u32x4 *>(S + ((size_t)a * BM2 + lrow + 128 * h) * LDK2 + lcol) = ra[a][h];
+#pragma unroll
+        for (int h = 0; h < BCH; ++h)
+            *reinterpret_cast<u32x4 *>(S + ((size_t)NA * BM2 + lrow + 128 * h) * LDK2 + lcol) = rb[h];
+    };
+
+    f32x16 acc[2][TN];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int nk = Kd / BK2;
+    const int fr = lane & 31, fk = (lane >> 5) * 8;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK2);
+        const bf16_t *S = smem + cur * STAGE;
+#pragma unroll
+        for (int kk = 0; kk < BK2; kk += 16) {
+            bf16x8 fa[NA][2], fb[TN];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                    fa[a][i] = *reinterpret_cast<const bf16x8 *>(S + ((size_t)a * BM2 + wm + i * 32 + fr) * LDK2 + kk + fk);
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+                fb[j] = *reinterpret_cast<const bf16x8 *>(S + ((size_t)NA * BM2 + wn + j * 32 + fr) * LDK2 + kk + fk);
+#pragma unroll
+            for (int a = NA - 1; a >= 0; --a)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a][i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m >= e.m_valid) continue;
+            const float add = EPI == EPI_LOGL ? e.bias[m] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int n = n0 + wn + j * 32 + (lane & 31);
+                if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = acc[i][j][r] + add;
+            }
+        }
+}
+
+template <int NA, int EPI, int BN2>
+hipError_t run2(int ntm, int ntn, hipStream_t st, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2, int64_t lda,
+                const bf16_t *B0, int64_t ldb, int Kd, const EpiArgs &e)
+{
+    hipLaunchKernelGGL((k_gemm_mfma2<NA, EPI, BN2>), dim3(ntm * ntn), dim3(512), 0, st, A0, A1, A2, lda, B0, ldb, Kd,
+                       ntn, e);
+    return hipGetLastError();
+}
+
+// Large-tile kernel for a single-term B operand: BN2 = 256 when N allows it and A has 2 terms.
+template <int EPI>
+hipError_t run_large(int M, int N, hipStream_t st, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
+                     int64_t lda, const bf16_t *B0, int64_t ldb, int Kd, const EpiArgs &e)
+{
+    const int ntm = M / BM2;
+    if (na == 2 && N % 256 == 0) return run2<2, EPI, 256>(ntm, N / 256, st, A0, A1, A2, lda, B0, ldb, Kd, e);
+    if (na == 2) return run2<2, EPI, 128>(ntm, N / 128, st, A0, A1, A2, lda, B0, ldb, Kd, e);
+    return run2<3, EPI, 128>(ntm, N / 128, st, A0, A1, A2, lda, B0, ldb, Kd, e);
+}
+
 }  // namespace
 
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
@@ -232,6 +365,8 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
     const dim3 grid(c->Ip / BN, c->chunkp / BM);
     const bf16_t *A1 = c->la_terms == 3 ? c->Lm : c->Ll;
     const bf16_t *A2 = c->Ll;
+    if (!c->k_split && c->opt.mfma_tile == 0)
+        return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->Pp, c->Pp, e);
     if (c->la_terms == 3) {
         if (c->k_split) return run<3, 2, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, c->K16lo, c->Pp, c->Pp, e);
         return run<3, 1, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);
@@ -263,6 +398,8 @@ hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipSt
         return run<2, 1, EPI_SCATTER>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);
     }
     if (c->k_split) return run<2, 2, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, c->KT16lo, c->Ip, c->Ip, e);
+    if (c->opt.mfma_tile == 0)
+        return run_large<EPI_STORE>(c->chunkp, c->Pp, st, 2, Ah, Al, nullptr, c->Ip, c->KT16, c->Ip, c->Ip, e);
     return run<2, 1, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);
 }
 

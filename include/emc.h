@@ -78,7 +78,8 @@ typedef struct emc_options {
     double blur_sigma;       /* model blur (voxels) after divide-by-weight; 0 = off (configs.h:80) */
     double fluence;          /* TRUE_POISSON photons per model unit; 0 = automatic per iteration */
     int32_t timing;          /* 1 = record per-stage HIP events (emc_iter_stats.stage_ms) */
-    int32_t reserved[7];
+    int32_t mfma_tile;       /* MFMA contraction tiling: 0 = 256-row tiles (default), 1 = 128x128 */
+    int32_t reserved[6];
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone
