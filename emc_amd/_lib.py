@@ -41,6 +41,22 @@ class IterStats(C.Structure):
                 ("model_avg", C.c_double), ("weight_sum", C.c_double), ("stage_ms", C.c_float * N_STAGES)]
 
 
+class Config(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("model_side", "read_stride", "detector_size", "rotations_n",
+                                          "sigma_half_life", "slice_chunk", "N_images", "max_iterations")] + \
+               [(n, C.c_double) for n in ("wavelength", "pixel_size", "detector_distance", "sigma_start",
+                                           "sigma_final")] + \
+               [(n, C.c_int32) for n in ("diff", "normalize_images", "known_intensity", "model_input",
+                                          "exclude_images", "output_period", "output_loop", "isOutput", "isDebug",
+                                          "blur_image", "nGPU")] + \
+               [(n, C.c_double) for n in ("blur_sigma", "exclude_ratio", "model_blur")] + \
+               [(n, C.c_char * 512) for n in ("image_prefix", "mask_file", "model_file")] + \
+               [(n, C.c_int32) for n in ("precision", "interp", "insert", "flat_detector")] + \
+               [("fluence", C.c_double), ("synth_atoms", C.c_int32)] + \
+               [(n, C.c_double) for n in ("synth_radius", "synth_photons", "beamstop", "outer_radius")] + \
+               [("synth_seed", C.c_uint64)]
+
+
 class EmcError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
@@ -93,6 +109,9 @@ SIGNATURES = {
     "emc_synth_patterns": (C.c_int, [f32p, C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, C.c_double, C.c_uint64,
                                      i32p, vp]),
     "emc_synth_uniform_model": (C.c_int, [C.c_int32, C.c_uint64, f32p]),
+    "emc_config_defaults": (C.c_int, [C.POINTER(Config)]),
+    "emc_config_parse": (C.c_int, [C.c_char_p, C.POINTER(Config)]),
+    "emc_config_read": (C.c_int, [C.c_char_p, C.POINTER(Config)]),
 }
 
 

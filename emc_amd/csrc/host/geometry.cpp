@@ -19,6 +19,12 @@ emc_status emc_host_fail(emc_status st, const char *msg)
     return st;
 }
 
+std::string emc_host_last_error_str()
+{
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err;
+}
+
 const char *emc_host_last_error()
 {
     std::lock_guard<std::mutex> lk(g_err_mu);

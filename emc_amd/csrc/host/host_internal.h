@@ -19,3 +19,7 @@ struct EmcRng {
     double uniform();        // [0, 1), 53 random bits
     int64_t poisson(double lam);
 };
+
+// Copy of the library-wide last error message.
+#include <string>
+std::string emc_host_last_error_str();

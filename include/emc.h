@@ -208,6 +208,37 @@ emc_status emc_synth_patterns(const float *model, int32_t L, const float *x, con
 /* Uniform random [0,1) model (model_input = 0, emc_helper.cpp:449-456, own RNG). */
 emc_status emc_synth_uniform_model(int32_t L, uint64_t seed, float *model);
 
+/* ---- configuration (emc.conf) ----------------------------------------------------------------- */
+
+/* Every key of the reference's Configuration (configs.h:12-86, read at configs.cpp:51-88) with
+ * its meaning, plus this engine's keys.  Strings are NUL-terminated. */
+typedef struct emc_config {
+    int32_t model_side, read_stride, detector_size, rotations_n, sigma_half_life, slice_chunk;
+    int32_t N_images, max_iterations;
+    double wavelength, pixel_size, detector_distance, sigma_start, sigma_final;
+    int32_t diff; /* from diff_type: "absolute", "poisson" (chi^2), "relative", "true_poisson" */
+    int32_t normalize_images, known_intensity, model_input, exclude_images;
+    int32_t output_period, output_loop, isOutput, isDebug, blur_image, nGPU;
+    double blur_sigma, exclude_ratio, model_blur;
+    char image_prefix[512], mask_file[512], model_file[512];
+    /* engine keys: precision "fp32"|"bf16x2"|"bf16x3", interp "nearest"|"trilinear",
+     * insert "atomic"|"fused"|"binned", fluence (0 = automatic), detector "flat"|"ewald" */
+    int32_t precision, interp, insert, flat_detector;
+    double fluence;
+    /* synthetic data (image_prefix = "synthetic"): atoms, sphere radius, photons per pattern,
+     * seed, mask beamstop and outer radius (pixels) */
+    int32_t synth_atoms;
+    double synth_radius, synth_photons, beamstop, outer_radius;
+    uint64_t synth_seed;
+} emc_config;
+
+/* Defaults for every key (the reference leaves missing keys uninitialised). */
+emc_status emc_config_defaults(emc_config *cfg);
+/* Config::read_configuration_file (configs.cpp:40-90): libconfig syntax subset; an unknown key,
+ * a wrong value type or a bad enum string is EMC_E_INVALID with the line in emc_last_error(NULL). */
+emc_status emc_config_parse(const char *text, emc_config *cfg);
+emc_status emc_config_read(const char *path, emc_config *cfg);
+
 #ifdef __cplusplus
 }
 #endif
