@@ -1,0 +1,139 @@
+// emc -- command-line driver of one-GPU EMC reconstruction over libemc.
+//
+//   emc [path/to/emc.conf]        (default ../emc.conf, as main.cpp:66-70)
+//
+// Replaces the reference's main.cpp loop (main.cpp:34-521) for one process on one GPU:
+// configuration (emc.conf keys), rotation list, detector coordinates, data, initial model, then
+// max_iterations EMC iterations.  Per iteration it prints one JSON line (iteration, sigma,
+// total logL, fluence, model average, stage times), appends "iteration rank seconds" to
+// ExeTime.data (file_helper.cpp:119-126) and, when isOutput is set, writes the model and weight
+// as raw little-endian float32 volumes output/model_%04d.bin and output/weight_%04d.bin
+// (the reference writes output/model_%04d.h5; HDF5 output is not part of this build).
+//
+// Data: image_prefix = "synthetic" generates the workload from the synth_* keys (Gaussian-atom
+// object, flat or Ewald detector, annular mask, Poisson patterns, uniform random initial model).
+#include <sys/stat.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../../include/emc.h"
+
+namespace {
+
+int die(const char *what, emc_status st, const emc_ctx *ctx)
+{
+    std::fprintf(stderr, "emc: %s failed (status %d): %s\n", what, (int)st, emc_last_error(ctx));
+    return 1;
+}
+
+bool write_raw(const char *path, const float *data, size_t n)
+{
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(data, sizeof(float), n, f) == n;
+    return std::fclose(f) == 0 && ok;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    const char *path = argc > 1 ? argv[1] : "../emc.conf";
+    emc_config cfg;
+    emc_status st = emc_config_read(path, &cfg);
+    if (st != EMC_OK) return die("reading the configuration", st, nullptr);
+    if (std::strcmp(cfg.image_prefix, "synthetic") != 0 && cfg.image_prefix[0] != '\0') {
+        std::fprintf(stderr, "emc: reading image files (%s) is not part of this build; set image_prefix = "
+                             "\"synthetic\"\n", cfg.image_prefix);
+        return 2;
+    }
+    const int L = cfg.model_side, P = L * L, I = cfg.N_images;
+    const int S = emc_rotation_count(cfg.rotations_n);
+    std::vector<float> q((size_t)S * 4), w(S);
+    if ((st = emc_generate_rotations(cfg.rotations_n, q.data(), w.data())) != EMC_OK) return die("rotations", st, nullptr);
+
+    std::vector<float> x(P), y(P), z(P);
+    st = cfg.flat_detector ? emc_coords_flat(L, x.data(), y.data(), z.data())
+                           : emc_coords_ewald(L, (float)cfg.pixel_size, (float)cfg.detector_distance,
+                                              (float)cfg.wavelength, x.data(), y.data(), z.data());
+    if (st != EMC_OK) return die("coordinates", st, nullptr);
+    std::vector<int32_t> mask(P);
+    if (emc_mask_annulus(L, cfg.beamstop, cfg.outer_radius, mask.data()) <= 0) return die("mask", EMC_E_INVALID, nullptr);
+
+    const size_t N3 = (size_t)L * L * L;
+    std::vector<float> truth(N3), model(N3), weight(N3);
+    std::vector<int32_t> K((size_t)I * P);
+    const uint64_t seed = cfg.synth_seed;
+    if ((st = emc_synth_model(L, cfg.synth_atoms, cfg.synth_radius, 1000 + seed, truth.data())) != EMC_OK)
+        return die("synthetic model", st, nullptr);
+    if ((st = emc_synth_patterns(truth.data(), L, x.data(), y.data(), z.data(), mask.data(), I, cfg.synth_photons,
+                                 2000 + seed, K.data(), nullptr)) != EMC_OK)
+        return die("synthetic patterns", st, nullptr);
+    if ((st = emc_synth_uniform_model(L, 3000 + seed, model.data())) != EMC_OK) return die("initial model", st, nullptr);
+
+    emc_dims dims = {L, S, I, cfg.slice_chunk};
+    emc_options opt;
+    emc_default_options(&opt);
+    opt.diff = cfg.diff;
+    opt.interp = cfg.interp;
+    opt.precision = cfg.diff == EMC_DIFF_TRUE_POISSON ? cfg.precision : EMC_PREC_FP32;
+    opt.insert = cfg.insert;
+    opt.sigma_start = cfg.sigma_start;
+    opt.sigma_final = cfg.sigma_final;
+    opt.sigma_half_life = cfg.sigma_half_life;
+    opt.blur_sigma = cfg.model_blur;
+    opt.fluence = cfg.fluence;
+    opt.timing = 1;
+    emc_ctx *ctx = nullptr;
+    if ((st = emc_create(&dims, &opt, &ctx)) != EMC_OK) return die("emc_create", st, nullptr);
+    int rc = 0;
+    auto guard = [&](emc_status s, const char *what) {
+        if (s != EMC_OK && rc == 0) rc = die(what, s, ctx);
+        return s == EMC_OK;
+    };
+    if (guard(emc_set_rotations(ctx, q.data(), w.data()), "set_rotations") &&
+        guard(emc_set_coords(ctx, x.data(), y.data(), z.data()), "set_coords") &&
+        guard(emc_set_mask(ctx, mask.data()), "set_mask") && guard(emc_set_patterns_i32(ctx, K.data()), "set_patterns") &&
+        guard(emc_set_model(ctx, model.data()), "set_model") && guard(emc_normalize_model(ctx, nullptr), "normalize")) {
+        FILE *tf = std::fopen("ExeTime.data", "wb");
+        if (cfg.isOutput) mkdir("output", 0755);
+        for (int it = 0; it < cfg.max_iterations && rc == 0; ++it) {
+            const auto t0 = std::chrono::steady_clock::now();
+            emc_iter_stats s;
+            if (!guard(emc_iterate(ctx, it, &s), "iterate")) break;
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::printf("{\"iteration\": %d, \"sigma\": %.6g, \"total_logl\": %.10g, \"fluence\": %.6g, "
+                        "\"model_avg\": %.6g, \"seconds\": %.6f, \"stage_ms\": {\"slice\": %.3f, \"logl\": %.3f, "
+                        "\"softmax\": %.3f, \"mstep\": %.3f, \"compress\": %.3f}}\n",
+                        it, s.sigma, s.total_logl, s.fluence, s.model_avg, sec, s.stage_ms[EMC_STAGE_SLICE],
+                        s.stage_ms[EMC_STAGE_LOGL], s.stage_ms[EMC_STAGE_SOFTMAX], s.stage_ms[EMC_STAGE_MSTEP],
+                        s.stage_ms[EMC_STAGE_COMPRESS]);
+            std::fflush(stdout);
+            if (tf) {
+                std::fprintf(tf, "%d %d   %e \n", it, 0, sec);
+                std::fflush(tf);
+            }
+            const bool out_now = cfg.isOutput && (!cfg.output_loop || cfg.output_period <= 0 || it % cfg.output_period == 0);
+            if (out_now) {
+                if (!guard(emc_get_model(ctx, model.data(), weight.data()), "get_model")) break;
+                char name[64];
+                std::snprintf(name, sizeof name, "output/model_%04d.bin", it);
+                bool ok = write_raw(name, model.data(), N3);
+                std::snprintf(name, sizeof name, "output/weight_%04d.bin", it);
+                ok = ok && write_raw(name, weight.data(), N3);
+                if (!ok) {
+                    std::fprintf(stderr, "emc: cannot write output files\n");
+                    rc = 1;
+                }
+            }
+        }
+        if (tf) std::fclose(tf);
+    }
+    emc_destroy(ctx);
+    return rc;
+}
