@@ -33,7 +33,8 @@ class Options(C.Structure):
     _fields_ = [("diff", C.c_int32), ("interp", C.c_int32), ("precision", C.c_int32), ("insert", C.c_int32),
                 ("sigma_start", C.c_double), ("sigma_final", C.c_double), ("sigma_half_life", C.c_int32),
                 ("device", C.c_int32), ("blur_sigma", C.c_double), ("fluence", C.c_double),
-                ("timing", C.c_int32), ("mfma_tile", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("timing", C.c_int32), ("mfma_tile", C.c_int32), ("count_bytes", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class IterStats(C.Structure):

@@ -164,6 +164,8 @@ extern "C" void emc_default_options(emc_options *o)
     o->blur_sigma = 0.0;
     o->fluence = 0.0;
     o->timing = 0;
+    o->mfma_tile = 0;
+    o->count_bytes = 0;
 }
 
 extern "C" const char *emc_last_error(const emc_ctx *c)
@@ -192,6 +194,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     if (c->stream) (void)hipSetDevice(c->opt.device);
     dfree(c->q); dfree(c->rotm); dfree(c->logw); dfree(c->coords); dfree(c->mask); dfree(c->axyz); dfree(c->pix);
     dfree(c->K32); dfree(c->KT32); dfree(c->K16); dfree(c->K16lo); dfree(c->KT16); dfree(c->KT16lo); dfree(c->sumK);
+    dfree(c->K8); dfree(c->KT8);
     dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
@@ -216,7 +219,8 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (d->model_side < 4 || d->model_side > 1024 || d->n_rotations < 1 || d->n_patterns < 1 || d->slice_chunk < 0)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad dimensions");
     if (opt.diff < 0 || opt.diff > 3 || opt.interp < 0 || opt.interp > 1 || opt.precision < 0 || opt.precision > 2 ||
-        opt.insert < 0 || opt.insert > 2)
+        opt.insert < 0 || opt.insert > 2 || (opt.count_bytes != 0 && opt.count_bytes != 2) || opt.mfma_tile < 0 ||
+        opt.mfma_tile > 1)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
     if (opt.precision != EMC_PREC_FP32 && opt.diff != EMC_DIFF_TRUE_POISSON)
         return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: MFMA precision modes need diff = TRUE_POISSON");
@@ -416,6 +420,26 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
         if (c->k_split && !c->K16lo) {
             HIP_TRY(c, dalloc(&c->K16lo, (int64_t)Ip * round_up(P, kPadP)));
             HIP_TRY(c, dalloc(&c->KT16lo, (int64_t)Ip * round_up(P, kPadP)));
+        }
+        // one byte per count when exact (<= 255) for the 256-row kernels
+        const bool u8 = c->opt.count_bytes == 0 && maxc <= 255.0;
+        if (u8) {
+            std::vector<uint8_t> b((size_t)Ip * Pp, 0), bT((size_t)Ip * Pp, 0);
+            for (int i = 0; i < I; ++i)
+                for (int j = 0; j < Pa; ++j) {
+                    const uint8_t k = (uint8_t)Kf[(size_t)i * P + pix[j]];
+                    b[(size_t)i * Pp + j] = k;
+                    bT[(size_t)j * Ip + i] = k;
+                }
+            if (!c->K8) {
+                HIP_TRY(c, dalloc(&c->K8, (int64_t)Ip * round_up(P, kPadP)));
+                HIP_TRY(c, dalloc(&c->KT8, (int64_t)Ip * round_up(P, kPadP)));
+            }
+            HIP_TRY(c, hipMemcpy(c->K8, b.data(), b.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->KT8, bT.data(), bT.size(), hipMemcpyHostToDevice));
+        } else {
+            dfree(c->K8);
+            dfree(c->KT8);
         }
         HIP_TRY(c, hipMemcpy(c->K16, hi.data(), sizeof(bf16_t) * hi.size(), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->KT16, hT.data(), sizeof(bf16_t) * hT.size(), hipMemcpyHostToDevice));

@@ -71,6 +71,7 @@ struct emc_ctx {
     float *KT32 = nullptr;    // [Pp][Ip]
     emc::bf16_t *K16 = nullptr, *K16lo = nullptr;    // [Ip][Pp] MFMA path
     emc::bf16_t *KT16 = nullptr, *KT16lo = nullptr;  // [Pp][Ip]
+    uint8_t *K8 = nullptr, *KT8 = nullptr;            // byte copies when every count <= 255
     double *sumK = nullptr;   // [Ip]
     // model
     float *model = nullptr, *Mnum = nullptr, *Wt = nullptr, *tmp3 = nullptr;  // [N3]

@@ -23,6 +23,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int BM = 128, BN = 128, BK = 64, LDK = BK + 8;
 
@@ -229,10 +230,24 @@ hipError_t run(dim3 grid, hipStream_t st, const bf16_t *A0, const bf16_t *A1, co
 // works on a contiguous band of tiles, which share A row panels in that XCD's L2.
 constexpr int BM2 = 256, BK2 = 32, LDK2 = BK2 + 8;
 
-template <int NA, int EPI, int BN2>
+// Counts stored as bytes (exact to 255) widen to BF16 on the way to LDS: 8 bytes -> 8 BF16.
+__device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
+{
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t w = v[k >> 1] >> ((k & 1) * 16);
+        const uint32_t f0 = __builtin_bit_cast(uint32_t, (float)(w & 0xffu));
+        const uint32_t f1 = __builtin_bit_cast(uint32_t, (float)((w >> 8) & 0xffu));
+        o[k] = (f0 >> 16) | (f1 & 0xffff0000u);  // integers < 256 have zero low mantissa bits
+    }
+    return o;
+}
+
+template <int NA, int EPI, int BN2, bool BU8>
 __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
                                                     const bf16_t *__restrict__ A2, int64_t lda,
-                                                    const bf16_t *__restrict__ B0, int64_t ldb, int Kd, int ntn,
+                                                    const void *__restrict__ Bv, int64_t ldb, int Kd, int ntn,
                                                     EpiArgs e)
 {
     constexpr int TN = BN2 / 64;                       // 32-wide tiles per wave along N
@@ -254,16 +269,21 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
     ga[0] = A0 + (size_t)(m0 + lrow) * lda + lcol;
     if (NA > 1) ga[1] = A1 + (size_t)(m0 + lrow) * lda + lcol;
     if (NA > 2) ga[NA - 1] = A2 + (size_t)(m0 + lrow) * lda + lcol;
-    const bf16_t *gb = B0 + (size_t)(n0 + lrow) * ldb + lcol;
+    // B element size 1 (bytes) or 2 (BF16); one chunk = 8 consecutive k
+    const unsigned char *gb = reinterpret_cast<const unsigned char *>(Bv) + ((size_t)(n0 + lrow) * ldb + lcol) * (BU8 ? 1 : 2);
 
     u32x4 ra[NA][2], rb[BCH];
+    u32x2 rb8[BCH];
     auto gload = [&](int k0) {
 #pragma unroll
         for (int a = 0; a < NA; ++a)
 #pragma unroll
             for (int h = 0; h < 2; ++h) ra[a][h] = *reinterpret_cast<const u32x4 *>(ga[a] + (size_t)128 * h * lda + k0);
 #pragma unroll
-        for (int h = 0; h < BCH; ++h) rb[h] = *reinterpret_cast<const u32x4 *>(gb + (size_t)128 * h * ldb + k0);
+        for (int h = 0; h < BCH; ++h) {
+            if (BU8) rb8[h] = *reinterpret_cast<const u32x2 *>(gb + (size_t)128 * h * ldb + k0);
+            else rb[h] = *reinterpret_cast<const u32x4 *>(gb + ((size_t)128 * h * ldb + k0) * 2);
+        }
     };
     auto sstore = [&](int stage) {
         bf16_t *S = smem + stage * STAGE;
@@ -274,7 +294,8 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
                 *reinterpret_cast<u32x4 *>(S + ((size_t)a * BM2 + lrow + 128 * h) * LDK2 + lcol) = ra[a][h];
 #pragma unroll
         for (int h = 0; h < BCH; ++h)
-            *reinterpret_cast<u32x4 *>(S + ((size_t)NA * BM2 + lrow + 128 * h) * LDK2 + lcol) = rb[h];
+            *reinterpret_cast<u32x4 *>(S + ((size_t)NA * BM2 + lrow + 128 * h) * LDK2 + lcol) =
+                BU8 ? widen_u8x8(rb8[h]) : rb[h];
     };
 
     f32x16 acc[2][TN];
@@ -332,24 +353,30 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
         }
 }
 
-template <int NA, int EPI, int BN2>
+template <int NA, int EPI, int BN2, bool BU8>
 hipError_t run2(int ntm, int ntn, hipStream_t st, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2, int64_t lda,
-                const bf16_t *B0, int64_t ldb, int Kd, const EpiArgs &e)
+                const void *B, int64_t ldb, int Kd, const EpiArgs &e)
 {
-    hipLaunchKernelGGL((k_gemm_mfma2<NA, EPI, BN2>), dim3(ntm * ntn), dim3(512), 0, st, A0, A1, A2, lda, B0, ldb, Kd,
-                       ntn, e);
+    hipLaunchKernelGGL((k_gemm_mfma2<NA, EPI, BN2, BU8>), dim3(ntm * ntn), dim3(512), 0, st, A0, A1, A2, lda, B, ldb,
+                       Kd, ntn, e);
     return hipGetLastError();
 }
 
-// Large-tile kernel for a single-term B operand: BN2 = 256 when N allows it and A has 2 terms.
+// Large-tile kernel for a single-term B operand (BF16, or bytes when B8 is given): BN2 = 256 when
+// N allows it and A has 2 terms.
 template <int EPI>
 hipError_t run_large(int M, int N, hipStream_t st, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
-                     int64_t lda, const bf16_t *B0, int64_t ldb, int Kd, const EpiArgs &e)
+                     int64_t lda, const bf16_t *B16, const uint8_t *B8, int64_t ldb, int Kd, const EpiArgs &e)
 {
     const int ntm = M / BM2;
-    if (na == 2 && N % 256 == 0) return run2<2, EPI, 256>(ntm, N / 256, st, A0, A1, A2, lda, B0, ldb, Kd, e);
-    if (na == 2) return run2<2, EPI, 128>(ntm, N / 128, st, A0, A1, A2, lda, B0, ldb, Kd, e);
-    return run2<3, EPI, 128>(ntm, N / 128, st, A0, A1, A2, lda, B0, ldb, Kd, e);
+    if (B8) {
+        if (na == 2 && N % 256 == 0) return run2<2, EPI, 256, true>(ntm, N / 256, st, A0, A1, A2, lda, B8, ldb, Kd, e);
+        if (na == 2) return run2<2, EPI, 128, true>(ntm, N / 128, st, A0, A1, A2, lda, B8, ldb, Kd, e);
+        return run2<3, EPI, 128, true>(ntm, N / 128, st, A0, A1, A2, lda, B8, ldb, Kd, e);
+    }
+    if (na == 2 && N % 256 == 0) return run2<2, EPI, 256, false>(ntm, N / 256, st, A0, A1, A2, lda, B16, ldb, Kd, e);
+    if (na == 2) return run2<2, EPI, 128, false>(ntm, N / 128, st, A0, A1, A2, lda, B16, ldb, Kd, e);
+    return run2<3, EPI, 128, false>(ntm, N / 128, st, A0, A1, A2, lda, B16, ldb, Kd, e);
 }
 
 }  // namespace
@@ -366,7 +393,8 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
     const bf16_t *A1 = c->la_terms == 3 ? c->Lm : c->Ll;
     const bf16_t *A2 = c->Ll;
     if (!c->k_split && c->opt.mfma_tile == 0)
-        return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->Pp, c->Pp, e);
+        return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->K8, c->Pp,
+                                   c->Pp, e);
     if (c->la_terms == 3) {
         if (c->k_split) return run<3, 2, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, c->K16lo, c->Pp, c->Pp, e);
         return run<3, 1, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);
@@ -399,7 +427,7 @@ hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipSt
     }
     if (c->k_split) return run<2, 2, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, c->KT16lo, c->Ip, c->Ip, e);
     if (c->opt.mfma_tile == 0)
-        return run_large<EPI_STORE>(c->chunkp, c->Pp, st, 2, Ah, Al, nullptr, c->Ip, c->KT16, c->Ip, c->Ip, e);
+        return run_large<EPI_STORE>(c->chunkp, c->Pp, st, 2, Ah, Al, nullptr, c->Ip, c->KT16, c->KT8, c->Ip, c->Ip, e);
     return run<2, 1, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);
 }
 

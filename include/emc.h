@@ -79,7 +79,9 @@ typedef struct emc_options {
     double fluence;          /* TRUE_POISSON photons per model unit; 0 = automatic per iteration */
     int32_t timing;          /* 1 = record per-stage HIP events (emc_iter_stats.stage_ms) */
     int32_t mfma_tile;       /* MFMA contraction tiling: 0 = 256-row tiles (default), 1 = 128x128 */
-    int32_t reserved[6];
+    int32_t count_bytes;     /* counts operand of the 256-row MFMA kernels: 0 = one byte per count when
+                                every count <= 255 (exact; widened to BF16 in registers), 2 = BF16 */
+    int32_t reserved[5];
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone

@@ -245,3 +245,22 @@ def test_invalid_inputs_are_rejected(small):
             e.set_mask(bad)
         with pytest.raises(emc_amd.EmcError):
             e.set_rotations(np.zeros((60, 4), np.float32), np.ones(60, np.float32))  # not unit
+
+
+def test_byte_counts_equal_bf16_counts(small):
+    """Counts stored as bytes and widened in registers give bit-identical logits and models."""
+    log("[count bytes]")
+    d = dict(small)
+    keep = d["K"].max(1) <= 255
+    d["K"] = d["K"][keep][:300]
+    outs = []
+    for cb in (0, 2):
+        with engine(d, "bf16x2", S=800, I=300, count_bytes=cb) as e:
+            e.logl()
+            R = e.get_logl()
+            e.normalize_resp()
+            e.maximize()
+            e.compress()
+            outs.append((R, e.get_model()[0]))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.allclose(outs[0][1], outs[1][1], rtol=1e-5, atol=1e-7)  # insertion order of float atomics varies
