@@ -85,6 +85,64 @@ def cpu_baseline(d, frac_rot: int):
                       f"({dt:.1f} s), extrapolated linearly in rotations"}
 
 
+def accuracy(d, n_sub_logl: int, n_model: int):
+    """Precision of the GPU paths against the FP64 oracle at this workload's rotations and model:
+    (a) logits / responsibilities of the first n_sub_logl patterns of the full run (each pattern's
+    logL and posterior depend only on itself once the fluence is fixed, so the oracle runs on the
+    subset with the GPU run's fluence); (b) the model after one iteration of a run on the first
+    n_model patterns (all of them enter the M-step)."""
+    import emc_amd
+    import oracle  # the checker
+    L, q, w = d["L"], d["q"], d["w"]
+    S = q.shape[0]
+    w64 = w.astype(np.float64)
+    m0 = d["model0"].astype(np.float64)
+    m0 = (m0 / m0[m0 > 0].mean()).astype(np.float32)
+    out = {}
+    for prec in ("fp32", "bf16x2", "bf16x3"):
+        I = d["K"].shape[0]
+        with emc_amd.Engine(L, S, I, precision=prec, insert="atomic" if prec == "fp32" else "binned") as e:
+            e.setup(q, w, d["coords"], d["mask"], d["K"], m0)
+            e.logl()
+            R = e.get_logl()[:, :n_sub_logl].copy()   # before normalize_resp: the FP32 path overwrites R
+            e.normalize_resp()
+            P = e.get_resp()[:, :n_sub_logl].copy()
+        log(f"[accuracy] {prec}: GPU logits done")
+        out[prec] = {"R": R, "P": P}
+    # oracle on the subset with the full run's fluence
+    Ks = d["K"][:n_sub_logl].astype(np.float64)
+    W = oracle.slices(m0.astype(np.float64), L, q, d["coords"], "trilinear")
+    phi0 = oracle.fluence(W, d["K"].astype(np.float64), d["mask"])
+    Ro, _ = oracle.logl(W, Ks, d["mask"], w64, "true_poisson", phi0=phi0)
+    del W
+    Po, lseo, _ = oracle.softmax(Ro)
+    log("[accuracy] oracle logits done")
+    res = {}
+    scale = np.maximum(np.abs(Ro).max(0, keepdims=True), 1.0)
+    for prec, v in out.items():
+        l1 = np.abs(v["P"].astype(np.float64) - Po).sum(0)
+        res[prec] = {"rel_logl": float(np.max(np.abs(v["R"] - Ro) / scale)),
+                     "max_abs_dlogl": float(np.abs(v["R"] - Ro).max()),
+                     "resp_l1_max": float(l1.max()), "resp_l1_mean": float(l1.mean())}
+    # model after one iteration on n_model patterns
+    Km = d["K"][:n_model]
+    ref = oracle.iterate(m0.astype(np.float64), L, q, w64, Km.astype(np.float64), d["mask"], d["coords"],
+                         "trilinear", "true_poisson", blur_sigma=d["cfg"]["blur"])
+    log("[accuracy] oracle iteration done")
+    for prec in ("fp32", "bf16x2", "bf16x3"):
+        with emc_amd.Engine(L, S, n_model, precision=prec, insert="atomic" if prec == "fp32" else "binned",
+                            blur_sigma=d["cfg"]["blur"]) as e:
+            e.setup(q, w, d["coords"], d["mask"], Km, m0)
+            st = e.iterate(0)
+            M, _ = e.get_model()
+        res[prec]["model_rel_l2"] = float(np.linalg.norm(M - ref["model"]) / np.linalg.norm(ref["model"]))
+        res[prec]["total_logl_rel"] = float(abs(st["total_logl"] - ref["total"]) / abs(ref["total"]))
+    res["definitions"] = (f"rel_logl = max |dlogL[s,i]| / max_s |logL[s,i]| and resp L1 = sum_s |dP[s,i]| over the "
+                          f"first {n_sub_logl} patterns of the full run; model_rel_l2 and total_logl_rel after one "
+                          f"iteration on the first {n_model} patterns; all against the FP64 oracle")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +155,7 @@ def main():
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
     ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
+    ap.add_argument("--accuracy", action="store_true", help="precision against the FP64 oracle (slow)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -141,6 +200,10 @@ def main():
             "algorithmic": f"2*I*S*P_act = {flops:.3e} FLOP per launch (one launch per iteration)",
             "mstep_achieved": flops / (st_mfma["mstep_gemm"] * 1e-3) / 1e12}
 
+    acc = accuracy(d, 400, 4000) if args.accuracy else None
+    if acc:
+        log(f"[bench] accuracy {json.dumps(acc)}")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -174,6 +237,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "variants": variants or None,
+        "accuracy": acc,
     }
     print(json.dumps(out))
 
