@@ -235,3 +235,72 @@ def iterate(model, L, q, w, K, mask, coords, interp="trilinear", diff="true_pois
                             _ptr(lse), _ptr(stats))
     return {"model": model, "Wt": Wt, "lse": lse, "total": tot, "P": Pm, "phi0": stats[0],
             "avg": stats[1]}
+
+
+# ---- orientation-proposal MLP (emc_oracle_mlp.c) ----------------------------------------------
+
+def _mlp_sigs(L):
+    L.orc_mlp_forward.argtypes = [f64p, C.c_int, C.c_int, C.c_int, C.c_int] + [f64p] * 6 + [f64p] * 6
+    L.orc_mlp_forward.restype = None
+    L.orc_mlp_loss.argtypes = [f64p, f64p, C.c_int, C.c_int]
+    L.orc_mlp_loss.restype = C.c_double
+    L.orc_mlp_backward.argtypes = [f64p, f64p, C.c_int, C.c_int, C.c_int, C.c_int] + [f64p] * 2 + [f64p] * 5 + \
+        [f64p] * 6
+    L.orc_mlp_backward.restype = None
+    L.orc_adam.argtypes = [f64p, f64p, f64p, f64p, C.c_long, C.c_int, C.c_double, C.c_double, C.c_double,
+                           C.c_double]
+    L.orc_adam.restype = None
+
+
+def mlp_forward(x, params):
+    """params: dict W1 [H,D], b1, W2 [H,H], b2, W3 [C,H], b3.  Returns dict of activations."""
+    L = lib()
+    if not hasattr(L, "_mlp"):
+        _mlp_sigs(L)
+        L._mlp = True
+    x = np.ascontiguousarray(x, np.float64)
+    B, D = x.shape
+    H = params["W1"].shape[0]
+    Cc = params["W3"].shape[0]
+    a = {k: np.zeros((B, H)) for k in ("pre1", "h1", "pre2", "h2")}
+    a["z"] = np.zeros((B, Cc))
+    a["p"] = np.zeros((B, Cc))
+    P = {k: np.ascontiguousarray(v, np.float64) for k, v in params.items()}
+    L.orc_mlp_forward(x, B, D, H, Cc, P["W1"], P["b1"], P["W2"], P["b2"], P["W3"], P["b3"], a["pre1"], a["h1"],
+                      a["pre2"], a["h2"], a["z"], a["p"])
+    return a
+
+
+def mlp_loss(p, t):
+    L = lib()
+    if not hasattr(L, "_mlp"):
+        _mlp_sigs(L)
+        L._mlp = True
+    p = np.ascontiguousarray(p, np.float64)
+    return L.orc_mlp_loss(p, np.ascontiguousarray(t, np.float64), p.shape[0], p.shape[1])
+
+
+def mlp_backward(x, t, params, acts):
+    L = lib()
+    if not hasattr(L, "_mlp"):
+        _mlp_sigs(L)
+        L._mlp = True
+    x = np.ascontiguousarray(x, np.float64)
+    B, D = x.shape
+    H = params["W1"].shape[0]
+    Cc = params["W3"].shape[0]
+    g = {k: np.zeros_like(np.asarray(v, np.float64)) for k, v in params.items()}
+    P = {k: np.ascontiguousarray(v, np.float64) for k, v in params.items()}
+    L.orc_mlp_backward(x, np.ascontiguousarray(t, np.float64), B, D, H, Cc, P["W2"], P["W3"], acts["pre1"],
+                       acts["h1"], acts["pre2"], acts["h2"], acts["p"], g["W1"], g["b1"], g["W2"], g["b2"],
+                       g["W3"], g["b3"])
+    return g
+
+
+def adam(w, g, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """In-place Adam step t (1-based) on float64 arrays."""
+    L = lib()
+    if not hasattr(L, "_mlp"):
+        _mlp_sigs(L)
+        L._mlp = True
+    L.orc_adam(w, np.ascontiguousarray(g, np.float64), m, v, w.size, t, lr, b1, b2, eps)
