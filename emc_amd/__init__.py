@@ -11,11 +11,12 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (DIFF, INSERT, INTERP, PRECISION, STAGES, Dims, EmcError, IterStats, Options, check, lib)
+from ._lib import (DIFF, INSERT, INTERP, PRECISION, SEARCH, STAGES, Dims, EmcError, IterStats, Options,
+                   ProposalOptions, check, lib)
 
 __all__ = ["Engine", "EmcError", "rotations", "rotation_count", "coords_flat", "coords_ewald", "mask_annulus",
            "sigma_schedule", "synth_object", "synth_model", "synth_patterns", "synth_uniform_model",
-           "default_options", "device_info"]
+           "default_options", "device_info", "proposal_options", "coarse_partition"]
 
 
 # ---- host utilities (no GPU) -------------------------------------------------------------------
@@ -121,6 +122,7 @@ class Engine:
 
     def __init__(self, L: int, S: int, I: int, slice_chunk: int = 0, **options):
         self.L, self.S, self.I = L, S, I
+        self.n_cells = self.top_k = self.hidden = 0  # set by enable_proposal
         self.opt = default_options(**options)
         d = Dims(L, S, I, slice_chunk)
         h = C.c_void_p()
@@ -242,4 +244,95 @@ class Engine:
         st = IterStats()
         self._c(lib().emc_iterate(self._h, iteration, C.byref(st)))
         return {"total_logl": st.total_logl, "sigma": st.sigma, "fluence": st.fluence, "model_avg": st.model_avg,
-                "weight_sum": st.weight_sum, "stage_ms": {k: float(st.stage_ms[i]) for i, k in enumerate(STAGES)}}
+                "weight_sum": st.weight_sum, "stage_ms": {k: float(st.stage_ms[i]) for i, k in enumerate(STAGES)},
+                "search": SEARCH.get(st.search, st.search), "n_fallback": st.n_fallback,
+                "predicted_mass": st.predicted_mass, "captured_mass": st.captured_mass,
+                "train_loss": st.train_loss, "work_fraction": st.work_fraction}
+
+    # ---- learned orientation proposal (no reference counterpart) ----
+
+    def enable_proposal(self, coarse_q, **options):
+        """Partition this engine's rotations into the cells of ``coarse_q`` [C][4] and set up the
+        network; options are fields of ``emc_proposal_options``."""
+        o = proposal_options(**options)
+        q = np.ascontiguousarray(coarse_q, np.float32)
+        self.n_cells = q.shape[0]
+        self.top_k = o.top_k
+        self.hidden = o.hidden
+        self._c(lib().emc_proposal_enable(self._h, q.reshape(-1), q.shape[0], C.byref(o)))
+
+    def disable_proposal(self):
+        self._c(lib().emc_proposal_disable(self._h))
+
+    def proposal_predict(self):
+        cells = np.zeros((self.I, self.top_k), np.int32)
+        mass = np.zeros(self.I, np.float32)
+        self._c(lib().emc_proposal_predict(self._h, cells.ctypes.data, mass.ctypes.data))
+        return cells, mass
+
+    def proposal_cell_targets(self):
+        t = np.zeros((self.I, self.n_cells), np.float32)
+        self._c(lib().emc_proposal_cell_targets(self._h, t))
+        return t
+
+    def proposal_train(self, epochs: int) -> float:
+        loss = C.c_double()
+        self._c(lib().emc_proposal_train(self._h, epochs, C.byref(loss)))
+        return loss.value
+
+    def proposal_features(self):
+        raw = np.zeros((self.I, 1024), np.float32)
+        self._c(lib().emc_proposal_features(self._h, raw))
+        return raw
+
+    def mlp_params(self):
+        H, Cn = self.hidden, self.n_cells
+        out = [np.zeros(s, np.float32) for s in ((H, 1024), H, (H, H), H, (Cn, H), Cn)]
+        self._c(lib().emc_mlp_get_params(self._h, *[a.ctypes.data for a in out]))
+        return out
+
+    def set_mlp_params(self, W1, b1, W2, b2, W3, b3):
+        arrs = [np.ascontiguousarray(a, np.float32).reshape(-1) for a in (W1, b1, W2, b2, W3, b3)]
+        self._c(lib().emc_mlp_set_params(self._h, *arrs))
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        probs = np.zeros((x.shape[0], self.n_cells), np.float32)
+        self._c(lib().emc_mlp_forward(self._h, x.reshape(-1), x.shape[0], probs))
+        return probs
+
+    def mlp_gradients(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        H, Cn = self.hidden, self.n_cells
+        g = [np.zeros(s, np.float32) for s in ((H, 1024), H, (H, H), H, (Cn, H), Cn)]
+        loss = C.c_double()
+        self._c(lib().emc_mlp_gradients(self._h, x.reshape(-1), t.reshape(-1), x.shape[0], C.byref(loss),
+                                        *[a.ctypes.data for a in g]))
+        return loss.value, g
+
+    def mlp_train_step(self, x, t) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        self._c(lib().emc_mlp_train_step(self._h, x.reshape(-1), t.reshape(-1), x.shape[0], C.byref(loss)))
+        return loss.value
+
+
+def proposal_options(**kw) -> ProposalOptions:
+    o = ProposalOptions()
+    lib().emc_proposal_defaults(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k) or k.startswith("reserved"):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def coarse_partition(fine_q, coarse_q):
+    """Cell of each fine rotation: argmax_c |q_s . q_c| (lowest c on ties)."""
+    f = np.ascontiguousarray(fine_q, np.float32)
+    q = np.ascontiguousarray(coarse_q, np.float32)
+    cell = np.zeros(f.shape[0], np.int32)
+    check(lib().emc_coarse_partition(f.reshape(-1), f.shape[0], q.reshape(-1), q.shape[0], cell))
+    return cell

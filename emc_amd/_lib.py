@@ -20,8 +20,9 @@ DIFF = {"absolute": 0, "poisson": 1, "relative": 2, "true_poisson": 3}
 INTERP = {"nearest": 0, "trilinear": 1}
 PRECISION = {"fp32": 0, "bf16x2": 1, "bf16x3": 2}
 INSERT = {"atomic": 0, "fused": 1, "binned": 2}
-N_STAGES = 8
-STAGES = ["slice", "logl", "softmax", "mstep", "compress", "total", "logl_gemm", "mstep_gemm"]
+N_STAGES = 10
+STAGES = ["slice", "logl", "softmax", "mstep", "compress", "total", "logl_gemm", "mstep_gemm", "propose", "train"]
+SEARCH = {0: "full", 1: "proposal"}
 
 
 class Dims(C.Structure):
@@ -39,7 +40,16 @@ class Options(C.Structure):
 
 class IterStats(C.Structure):
     _fields_ = [("total_logl", C.c_double), ("sigma", C.c_double), ("fluence", C.c_double),
-                ("model_avg", C.c_double), ("weight_sum", C.c_double), ("stage_ms", C.c_float * N_STAGES)]
+                ("model_avg", C.c_double), ("weight_sum", C.c_double), ("stage_ms", C.c_float * N_STAGES),
+                ("search", C.c_int32), ("n_fallback", C.c_int32), ("predicted_mass", C.c_double),
+                ("captured_mass", C.c_double), ("train_loss", C.c_double), ("work_fraction", C.c_double)]
+
+
+class ProposalOptions(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("top_k", "warmup_full", "full_every", "hidden", "epochs", "batch",
+                                          "synth_patterns", "reserved0")] + \
+               [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "reserved1")] + \
+               [("seed", C.c_uint64)]
 
 
 class Config(C.Structure):
@@ -113,6 +123,19 @@ SIGNATURES = {
     "emc_config_defaults": (C.c_int, [C.POINTER(Config)]),
     "emc_config_parse": (C.c_int, [C.c_char_p, C.POINTER(Config)]),
     "emc_config_read": (C.c_int, [C.c_char_p, C.POINTER(Config)]),
+    "emc_proposal_defaults": (None, [C.POINTER(ProposalOptions)]),
+    "emc_coarse_partition": (C.c_int, [f32p, C.c_int32, f32p, C.c_int32, i32p]),
+    "emc_proposal_enable": (C.c_int, [ctxp, f32p, C.c_int32, C.POINTER(ProposalOptions)]),
+    "emc_proposal_disable": (C.c_int, [ctxp]),
+    "emc_proposal_predict": (C.c_int, [ctxp, vp, vp]),
+    "emc_proposal_cell_targets": (C.c_int, [ctxp, f32p]),
+    "emc_proposal_train": (C.c_int, [ctxp, C.c_int32, C.POINTER(C.c_double)]),
+    "emc_proposal_features": (C.c_int, [ctxp, f32p]),
+    "emc_mlp_get_params": (C.c_int, [ctxp, vp, vp, vp, vp, vp, vp]),
+    "emc_mlp_set_params": (C.c_int, [ctxp, f32p, f32p, f32p, f32p, f32p, f32p]),
+    "emc_mlp_forward": (C.c_int, [ctxp, f32p, C.c_int32, f32p]),
+    "emc_mlp_gradients": (C.c_int, [ctxp, f32p, f32p, C.c_int32, C.POINTER(C.c_double), vp, vp, vp, vp, vp, vp]),
+    "emc_mlp_train_step": (C.c_int, [ctxp, f32p, f32p, C.c_int32, C.POINTER(C.c_double)]),
 }
 
 

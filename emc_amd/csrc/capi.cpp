@@ -192,6 +192,12 @@ extern "C" void emc_destroy(emc_ctx *c)
 {
     if (!c) return;
     if (c->stream) (void)hipSetDevice(c->opt.device);
+    if (c->prop) {
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        proposal_free(c->prop);
+        delete c->prop;
+        c->prop = nullptr;
+    }
     dfree(c->q); dfree(c->rotm); dfree(c->logw); dfree(c->coords); dfree(c->mask); dfree(c->axyz); dfree(c->pix);
     dfree(c->K32); dfree(c->KT32); dfree(c->K16); dfree(c->K16lo); dfree(c->KT16); dfree(c->KT16lo); dfree(c->sumK);
     dfree(c->K8); dfree(c->KT8);
@@ -336,6 +342,7 @@ extern "C" emc_status emc_set_rotations(emc_ctx *c, const float *q, const float 
     HIP_TRY(c, e);
     c->has_rot = true;
     c->has_logl = c->has_resp = false;
+    EMC_TRY(emc_proposal_disable(c));  // its cell partition belonged to the old rotations
     return refresh_bins(c);
 }
 
@@ -376,11 +383,13 @@ extern "C" emc_status emc_set_mask(emc_ctx *c, const int32_t *mask)
     c->has_mask = true;
     c->has_K = false;  // compaction changed
     c->has_logl = c->has_resp = false;
+    EMC_TRY(emc_proposal_disable(c));  // its pattern features used the old compaction
     return refresh_bins(c);
 }
 
 static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
 {
+    EMC_TRY(emc_proposal_disable(c));  // its features belonged to the old patterns
     const int I = c->I, Ip = c->Ip, Pp = c->Pp, Pa = c->Pa, P = c->P;
     std::vector<int32_t> pix(Pp);
     HIP_TRY(c, hipMemcpy(pix.data(), c->pix, sizeof(int32_t) * Pp, hipMemcpyDeviceToHost));
@@ -603,7 +612,8 @@ extern "C" emc_status emc_get_slice_weights(emc_ctx *c, double *T)
 
 // ---- stages ----------------------------------------------------------------------------------
 
-static emc_status logl_enqueue(emc_ctx *c, float sigma)
+// Slicing statistics, fluence, per-rotation bias and per-pattern constants (all rotations).
+static emc_status slice_prepare(emc_ctx *c, float sigma)
 {
     hipStream_t st = c->stream;
     ev_start(c, EMC_STAGE_SLICE);
@@ -611,6 +621,13 @@ static emc_status logl_enqueue(emc_ctx *c, float sigma)
     HIP_TRY(c, launch_bias(c, sigma, st));
     HIP_TRY(c, launch_pattern_const(c, c->mfma, st));
     ev_stop(c, EMC_STAGE_SLICE);
+    return EMC_OK;
+}
+
+static emc_status logl_enqueue(emc_ctx *c, float sigma)
+{
+    hipStream_t st = c->stream;
+    EMC_TRY(slice_prepare(c, sigma));
     ev_start(c, EMC_STAGE_LOGL);
     for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
         const int cs = std::min(c->chunk, c->S - s0);
@@ -785,6 +802,45 @@ extern "C" emc_status emc_normalize_model(emc_ctx *c, double *model_avg)
     return EMC_OK;
 }
 
+// ---- proposal iteration stages (the dense stages restricted to each pattern's selected cells) --
+
+static emc_status sparse_logl_enqueue(emc_ctx *c, float sigma)
+{
+    hipStream_t st = c->stream;
+    EMC_TRY(slice_prepare(c, sigma));
+    ev_start(c, EMC_STAGE_LOGL);
+    HIP_TRY(c, launch_slice_bf16(c, 0, c->S, st));
+    gev_rec(c, 0, 0, 0);
+    HIP_TRY(c, launch_grouped_logl(c, st));
+    gev_rec(c, 0, 0, 1);
+    ev_stop(c, EMC_STAGE_LOGL);
+    return EMC_OK;
+}
+
+static emc_status sparse_softmax_enqueue(emc_ctx *c)
+{
+    ev_start(c, EMC_STAGE_SOFTMAX);
+    HIP_TRY(c, launch_grouped_softmax(c, c->stream));
+    ev_stop(c, EMC_STAGE_SOFTMAX);
+    return EMC_OK;
+}
+
+static emc_status sparse_mstep_enqueue(emc_ctx *c)
+{
+    hipStream_t st = c->stream;
+    ev_start(c, EMC_STAGE_MSTEP);
+    HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
+    HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
+    HIP_TRY(c, launch_build_ktg(c, st));
+    gev_rec(c, 1, 0, 0);
+    HIP_TRY(c, launch_grouped_mstep(c, st));
+    gev_rec(c, 1, 0, 1);
+    if (c->opt.insert == EMC_INSERT_BINNED) HIP_TRY(c, launch_insert_binned(c, 0, c->S, st));
+    else HIP_TRY(c, launch_insert(c, 0, c->S, st));
+    ev_stop(c, EMC_STAGE_MSTEP);
+    return EMC_OK;
+}
+
 extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats *stats)
 {
     if (!c) return fail(c, EMC_E_INVALID, "iterate: null context");
@@ -795,14 +851,55 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
     const double sigma = emc_sigma_schedule(iteration, c->opt.sigma_start, c->opt.sigma_final, c->opt.sigma_half_life);
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL], c->stream);
     HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
-    EMC_TRY(logl_enqueue(c, (float)sigma));
-    EMC_TRY(softmax_enqueue(c));
-    EMC_TRY(mstep_enqueue(c));
+
+    // search policy: full search until the network has trained, during the warm-up, on every
+    // full_every-th iteration after it, and when too many patterns would fall back
+    Proposal *pr = c->prop;
+    bool full = true, predicted = false;
+    double captured = -1.0, train_loss = -1.0, pmass_mean = 0.0;
+    int n_fb = 0;
+    if (pr) {
+        const int t = pr->iter, w = pr->o.warmup_full;
+        full = pr->passes == 0 || t < w || (t - w + 1) % pr->o.full_every == 0;
+        if (pr->passes > 0) {
+            ev_start(c, EMC_STAGE_PROPOSE);
+            EMC_TRY(proposal_predict(c));
+            predicted = true;
+            if (!full) {
+                bool too_many = false;
+                EMC_TRY(proposal_build_lists(c, &too_many));
+                if (too_many) full = true;
+            }
+            ev_stop(c, EMC_STAGE_PROPOSE);
+            double s = 0.0;
+            for (int i = 0; i < c->I; ++i) {
+                s += pr->pmass_h[i];
+                n_fb += !(pr->pmass_h[i] >= pr->o.min_mass);
+            }
+            pmass_mean = s / c->I;
+        }
+    }
+    if (full) {
+        EMC_TRY(logl_enqueue(c, (float)sigma));
+        EMC_TRY(softmax_enqueue(c));
+        EMC_TRY(mstep_enqueue(c));
+        if (pr) {
+            ev_start(c, EMC_STAGE_TRAIN);
+            EMC_TRY(proposal_targets(c, predicted, &captured));
+            EMC_TRY(proposal_train(c, pr->o.epochs, &train_loss));
+            ev_stop(c, EMC_STAGE_TRAIN);
+        }
+    } else {
+        EMC_TRY(sparse_logl_enqueue(c, (float)sigma));
+        EMC_TRY(sparse_softmax_enqueue(c));
+        EMC_TRY(sparse_mstep_enqueue(c));
+    }
     EMC_TRY(compress_enqueue(c));
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL + 1], c->stream);
     EMC_TRY(sync_scalars(c));
-    c->has_logl = c->mfma;
-    c->has_resp = true;
+    if (pr) ++pr->iter;
+    c->has_logl = full && c->mfma;
+    c->has_resp = full;  // a proposal iteration leaves per-cell blocks, not the [S][I] matrix
     c->has_mstep = true;
     if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "iterate: non-finite log-sum-exp or model normaliser");
     if (stats) {
@@ -812,6 +909,12 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         stats->model_avg = c->scal_h[SC_AVG];
         stats->weight_sum = c->scal_h[SC_WSUM];
         EMC_TRY(total_logl_host(c, &stats->total_logl));
+        stats->search = full ? EMC_SEARCH_FULL : EMC_SEARCH_PROPOSAL;
+        stats->n_fallback = predicted ? n_fb : 0;
+        stats->predicted_mass = pmass_mean;
+        stats->captured_mass = captured;
+        stats->train_loss = train_loss;
+        stats->work_fraction = full ? 1.0 : pr->work_fraction;
         if (c->opt.timing) {
             const int stages[] = {EMC_STAGE_SLICE, EMC_STAGE_LOGL, EMC_STAGE_SOFTMAX, EMC_STAGE_MSTEP, EMC_STAGE_COMPRESS,
                                   EMC_STAGE_TOTAL};
@@ -821,6 +924,16 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
             }
             stats->stage_ms[EMC_STAGE_LOGL_GEMM] = gev_sum(c, 0);
             stats->stage_ms[EMC_STAGE_MSTEP_GEMM] = gev_sum(c, 1);
+            if (predicted) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, c->ev[2 * EMC_STAGE_PROPOSE], c->ev[2 * EMC_STAGE_PROPOSE + 1]) == hipSuccess)
+                    stats->stage_ms[EMC_STAGE_PROPOSE] = ms;
+            }
+            if (pr && full) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, c->ev[2 * EMC_STAGE_TRAIN], c->ev[2 * EMC_STAGE_TRAIN + 1]) == hipSuccess)
+                    stats->stage_ms[EMC_STAGE_TRAIN] = ms;
+            }
         }
     }
     return EMC_OK;

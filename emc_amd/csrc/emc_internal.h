@@ -32,7 +32,73 @@ enum {
     SC_NONFINITE = 8, // > 0 if a non-finite lse or normaliser appeared
     SC_SUMW = 9,      // sum over rotations of the slice sums
     SC_B0 = 10,       // global logit shift removed from the stored logits (true Poisson)
+    SC_LOSS = 11,     // proposal network: summed cross-entropy of a training pass
     SC_N = 16
+};
+
+// Learned orientation proposal: network, training state and the grouped (per coarse cell) work
+// lists of a proposal iteration.
+struct Proposal {
+    emc_proposal_options o{};
+    int C = 0, Cp = 0, H = 0;           // cells, cells padded to 64, hidden width
+    static constexpr int D = 1024;      // features: 32 x 32 pooled pattern
+    int iter = 0;                       // iterations run since enable
+    int passes = 0;                     // training passes run
+    int adam_t = 0;                     // Adam step counter
+    double last_loss = -1.0;
+    // partition of the fine rotations
+    std::vector<int32_t> cell_h, rbeg_h, cell_rot_h;  // [S], [C+1], [S]
+    int32_t *cell_of = nullptr, *cell_rot = nullptr, *rbeg = nullptr;
+    // parameters: one FP32 vector W1 | b1 | W2 | b2 | W3 [Cp][H] | b3 [Cp] (rows >= C zero)
+    int64_t n_par = 0, oW1 = 0, ob1 = 0, oW2 = 0, ob2 = 0, oW3 = 0, ob3 = 0;
+    float *par = nullptr, *grad = nullptr, *am = nullptr, *av = nullptr;
+    uint16_t *par16 = nullptr;
+    // features
+    float *mu = nullptr, *istd = nullptr;  // [D]
+    uint16_t *X = nullptr;                 // [Ip][D] standardised features of the patterns
+    // activations, rows_cap rows (inference runs over all patterns at once)
+    int rows_cap = 0;
+    float *pre1 = nullptr, *pre2 = nullptr, *z = nullptr, *prob = nullptr;
+    uint16_t *h1 = nullptr, *h2 = nullptr, *dz = nullptr, *d1 = nullptr, *d2 = nullptr;
+    double *loss_part = nullptr;
+    // training
+    float *Tt = nullptr;           // [Cp][Ip] cell sums of the responsibilities
+    bool targets_valid = false;
+    uint16_t *Xb = nullptr;        // [batch][D]
+    float *tb = nullptr;           // [batch][Cp]
+    int32_t *bidx = nullptr;       // [I + n_syn] shuffled training indices
+    int n_syn = 0;
+    float *syn_counts = nullptr;   // [n_syn][Pp]
+    uint16_t *Xs = nullptr;        // [n_syn][D]
+    int32_t *syn_rot = nullptr;    // [n_syn]
+    uint64_t rng_state = 0;
+    // prediction
+    int32_t *topc = nullptr;       // [Ip][top_k]
+    float *pmass = nullptr;        // [Ip]
+    double *cap = nullptr;         // [Ip] captured mass per pattern
+    std::vector<int32_t> topc_h;
+    std::vector<float> pmass_h;
+    bool pred_valid = false;
+    // grouped work lists (built on the host from the prediction)
+    int64_t n_slots = 0, n_gslots = 0, n_blk = 0;  // slots, padded slots, block elements
+    int n_fallback = 0;
+    double work_fraction = 0.0;
+    int n_tiles_e = 0, n_tiles_m = 0, n_tiles_sm = 0, n_tiles_kt = 0;
+    int64_t list_cap = 0;          // int32 elements of `lists`
+    int32_t *lists = nullptr;      // device copy of the packed lists below
+    int32_t *lists_h = nullptr;    // pinned host staging
+    int64_t *boff = nullptr;       // [C] element offset of each cell's logits / P block
+    int64_t *boff_h = nullptr;     // pinned host staging
+    float *raw = nullptr;          // [max(Ip, n_syn)][D] unstandardised features (scratch)
+    // offsets (in int32 elements) of each list inside `lists`
+    int64_t off_slot_pat = 0, off_sbeg = 0, off_nsp = 0, off_pbeg = 0, off_pslot = 0, off_tile_e = 0,
+            off_tile_m = 0, off_tile_sm = 0, off_tile_kt = 0, off_boff = 0, off_gbeg = 0;
+    int64_t ktg_cap = 0;           // elements of ktg
+    void *ktg = nullptr;           // [sum nsp_c][Pp]... stored per cell as [Pp][nsp_c]
+    int64_t slot_cap = 0;          // doubles of slot scratch
+    double *slot_ps = nullptr;     // [slots] partial sums
+    float *slot_pm = nullptr;      // [slots] partial maxima
+    double *slot_lse = nullptr;    // [slots]
 };
 
 }  // namespace emc
@@ -101,6 +167,8 @@ struct emc_ctx {
     int32_t *bin_units = nullptr;                      // [n_units][3] = key, begin, end
     std::vector<int> bin_chunk_units;                  // unit range of each rotation chunk
     bool bins_valid = false;
+    // learned proposal (null when disabled)
+    emc::Proposal *prop = nullptr;
 };
 
 namespace emc {
@@ -142,6 +210,50 @@ hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st
 hipError_t launch_divide(const emc_ctx *c, hipStream_t st);
 hipError_t launch_blur(const emc_ctx *c, hipStream_t st);
 hipError_t launch_normalize(const emc_ctx *c, bool apply, hipStream_t st);
+// Deterministic single-workgroup sum of n doubles into *out.
+hipError_t launch_sum_f64(const double *x, int n, double *out, hipStream_t st);
+
+// ---- learned proposal ------------------------------------------------------------------------
+// Grouped (per coarse cell) contractions and softmax over the work lists in c->prop.
+hipError_t launch_grouped_logl(const emc_ctx *c, hipStream_t st);
+hipError_t launch_grouped_softmax(const emc_ctx *c, hipStream_t st);
+hipError_t launch_build_ktg(const emc_ctx *c, hipStream_t st);
+hipError_t launch_grouped_mstep(const emc_ctx *c, hipStream_t st);
+
+// Network kernels.  mlp_gemm: C[m,n] = sum_k A(m,k) B(n,k), A/B "col" = stored [k][m] / [k][n];
+// M, N multiples of 64, K of 32.
+enum { MLP_EPI_STORE = 0, MLP_EPI_BIAS_GELU = 1, MLP_EPI_BIAS = 2, MLP_EPI_GELU_GRAD = 3 };
+hipError_t mlp_gemm(bool acol, bool bcol, int epi, const bf16_t *A, int lda, const bf16_t *B, int ldb, int M, int N,
+                    int K, float *out32, bf16_t *out16, int ldo, const float *bias, const float *pre, int m_valid,
+                    int n_valid, hipStream_t st);
+hipError_t mlp_features(const float *K32, const bf16_t *K16, const uint8_t *K8, int ld, int Pa, const int32_t *pix,
+                        int L, int rows, int rows_pad, float *raw, hipStream_t st);
+hipError_t mlp_feat_stats(const float *raw, int rows, float *mu, float *istd, hipStream_t st);
+hipError_t mlp_standardise(const float *raw, int rows, int rows_pad, const float *mu, const float *istd, bf16_t *X,
+                           hipStream_t st);
+hipError_t mlp_softmax_ce(const float *z, int Cp, int C, int rows_valid, int rows, const float *t, float scale,
+                          float *p, bf16_t *dz, double *loss_part, hipStream_t st);
+hipError_t mlp_accum(const double *x, int n, double *acc, hipStream_t st);
+hipError_t mlp_colsum(const bf16_t *d, int rows, int ld, int n_valid, float *g, hipStream_t st);
+hipError_t mlp_adam(float *w, const float *g, float *m, float *v, bf16_t *w16, int64_t n, float lr, int t,
+                    hipStream_t st);
+hipError_t mlp_to_bf16(const float *w, int64_t n, bf16_t *w16, hipStream_t st);
+hipError_t mlp_topk(const float *p, int Cp, int C, int rows, int k, int32_t *cells, float *mass, hipStream_t st);
+hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, hipStream_t st);
+hipError_t mlp_captured(const emc_ctx *c, double *cap, hipStream_t st);
+hipError_t mlp_gather_batch(const emc_ctx *c, const int32_t *idx, int rows, hipStream_t st);
+hipError_t mlp_synth_counts(const emc_ctx *c, int n, float photons, uint64_t seed, hipStream_t st);
+
+// Host side of the proposal (proposal.cpp); each sets c->err on failure.
+void proposal_free(Proposal *p);
+// Network prediction for every pattern into topc / pmass (device) and topc_h / pmass_h.
+emc_status proposal_predict(emc_ctx *c);
+// Grouped work lists from the prediction; *too_many = the fallback share exceeds max_fallback.
+emc_status proposal_build_lists(emc_ctx *c, bool *too_many);
+// Cell targets from the dense responsibilities; captured mass of the last prediction if asked.
+emc_status proposal_targets(emc_ctx *c, bool measure, double *captured);
+// Training passes over the targets plus synthetic patterns; loss of the last pass.
+emc_status proposal_train(emc_ctx *c, int epochs, double *loss);
 
 // BF16 helpers (host and device agree: round to nearest even, finite inputs).
 __host__ __device__ inline bf16_t f2bf(float x)

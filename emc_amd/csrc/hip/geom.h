@@ -51,4 +51,46 @@ __device__ __forceinline__ bool tri_axis(float u, int L, int &b, float &f)
 
 __device__ __forceinline__ float model_c(int L) { return 0.5f * (float)L - 0.5f; }
 
+// Slice value of rotation matrix m at detector point (X,Y,Z); -1 when invalid.
+template <int INTERP>
+__device__ __forceinline__ float sample(const float *__restrict__ model, int L, const float *m,
+                                        float X, float Y, float Z)
+{
+    float ux, uy, uz;
+    rot_point(m, X, Y, Z, model_c(L), ux, uy, uz);
+    if (INTERP == 0) {
+        // nearest: lroundf and the exclusive lower bound of emc_cuda.cu:129-138
+        const int vx = nearest(ux), vy = nearest(uy), vz = nearest(uz);
+        if (vx > 0 && vx < L && vy > 0 && vy < L && vz > 0 && vz < L)
+            return model[((size_t)vz * L + vy) * L + vx];
+        return -1.0f;
+    } else {
+        int bx, by, bz;
+        float fx, fy, fz;
+        if (!tri_axis(ux, L, bx, fx) || !tri_axis(uy, L, by, fy) || !tri_axis(uz, L, bz, fz))
+            return -1.0f;
+        const float *b = model + ((size_t)bz * L + by) * L + bx;
+        const size_t LL = (size_t)L * L;
+        const float v000 = b[0], v001 = b[1], v010 = b[L], v011 = b[L + 1];
+        const float v100 = b[LL], v101 = b[LL + 1], v110 = b[LL + L], v111 = b[LL + L + 1];
+        const float mn = fminf(fminf(fminf(v000, v001), fminf(v010, v011)),
+                               fminf(fminf(v100, v101), fminf(v110, v111)));
+        if (mn < 0.0f) return -1.0f;  // a missing voxel among the corners
+        const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+        const float c00 = v000 * gx + v001 * fx, c01 = v010 * gx + v011 * fx;
+        const float c10 = v100 * gx + v101 * fx, c11 = v110 * gx + v111 * fx;
+        const float c0 = c00 * gy + c01 * fy, c1 = c10 * gy + c11 * fy;
+        return c0 * gz + c1 * fz;
+    }
+}
+
+__device__ __forceinline__ void load_rot(const float *__restrict__ rotm, int s, float *m)
+{
+    const float4 a = *reinterpret_cast<const float4 *>(rotm + 12 * (size_t)s);
+    const float4 b = *reinterpret_cast<const float4 *>(rotm + 12 * (size_t)s + 4);
+    const float c = rotm[12 * (size_t)s + 8];
+    m[0] = a.x; m[1] = a.y; m[2] = a.z; m[3] = a.w;
+    m[4] = b.x; m[5] = b.y; m[6] = b.z; m[7] = b.w; m[8] = c;
+}
+
 }  // namespace emc

@@ -1,0 +1,425 @@
+// Grouped contractions of a proposal iteration: each pattern is scored only against the fine
+// rotations of its selected coarse cells.
+//
+// No reference counterpart (main.cpp:239-480 scores every rotation for every pattern); each
+// group is the dense stage restricted to one cell, so the arithmetic per (rotation, pattern)
+// pair is the dense MFMA path's (same operands, same k order, same epilogue).
+//
+// Work lists (built on the host per iteration): cell c owns the fine rotations
+// cell_rot[rbeg[c] .. rbeg[c+1]) (nr_c of them) and the "slots" sbeg[c] .. sbeg[c+1) (ns_c
+// patterns that selected c; slot_pat gives the pattern).  Per cell, logits and responsibilities
+// are a row-major block [nr_c][nsp_c] (nsp_c = ns_c rounded up to 32, padding columns hold P = 0)
+// at element offset boff[c] of c->R (logits) and c->Ph / c->Pl (P hi / lo).  The M-step reads the
+// cell's counts transposed and gathered, KTg block [Pp][nsp_c] at element gbeg[c] * Pp.
+//
+//   logL   : block(c)[r][n] = sum_j logW~[rot(r), j] K[pat(n), j] + bias[rot(r)]    (rows gathered)
+//   M-step : U[rot(r), j]   = sum_n P(c)[r][n] KTg(c)[j][n]
+// Tiles are 128 x 128, 256 threads = 4 wavefronts each 64 x 64 (2 x 2 v_mfma_f32_32x32x16_bf16),
+// BK = 32 with two LDS stages (rows padded to 40 BF16), workgroups renumbered per XCD.
+#include <algorithm>
+
+#include "../emc_internal.h"
+
+namespace emc {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+constexpr int TM = 128, TN = 128, TK = 32, TLD = TK + 8;
+
+struct GTab {
+    const int32_t *cell_rot, *rbeg, *cell_of;           // partition of the fine rotations
+    const int32_t *slot_pat, *sbeg, *nsp, *gbeg;         // slots per cell
+    const int32_t *pbeg, *pslot;                         // slots per pattern
+    const int64_t *boff;                                 // block offsets
+    const int4 *tiles;                                   // (cell, m0, n0, -)
+};
+
+__device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
+{
+    u32x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t w = v[k >> 1] >> ((k & 1) * 16);
+        const uint32_t f0 = __builtin_bit_cast(uint32_t, (float)(w & 0xffu));
+        const uint32_t f1 = __builtin_bit_cast(uint32_t, (float)((w >> 8) & 0xffu));
+        o[k] = (f0 >> 16) | (f1 & 0xffff0000u);  // integers < 256 have zero low mantissa bits
+    }
+    return o;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+enum { G_LOGL = 0, G_MSTEP = 1 };
+
+// MODE G_LOGL : A = NA log-slice terms [S][lda] (rows gathered by cell_rot), B = counts [I][ldb]
+//               (rows gathered by slot_pat), k extent Kd = Pp.
+// MODE G_MSTEP: A = P hi, lo blocks (NA = 2), B = KTg block, k extent nsp_c.
+template <int MODE, int NA, bool BU8>
+__global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                                 const bf16_t *__restrict__ A2, int64_t lda,
+                                                 const void *__restrict__ Bv, int64_t ldb, int Pp, GTab g,
+                                                 const float *__restrict__ bias, float *__restrict__ out)
+{
+    constexpr int STAGE = (NA * TM + TN) * TLD;
+    __shared__ __attribute__((aligned(16))) bf16_t smem[2 * STAGE];
+    __shared__ float sbias[TM];
+    __shared__ int srot[TM];
+
+    const int nwg = gridDim.x, bid = blockIdx.x;
+    const int base = nwg / 8, rem = nwg % 8, xcd = bid % 8, loc = bid / 8;
+    const int t = xcd * base + min(xcd, rem) + loc;
+    const int4 td = g.tiles[t];
+    const int c = td.x, m0 = td.y, n0 = td.z;
+    const int rb = g.rbeg[c], nr = g.rbeg[c + 1] - rb;
+    const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
+    const int64_t boff = g.boff[c];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int lrow = tid >> 2, lcol = (tid & 3) * 8;
+
+    if (tid < TM) {
+        const int rot = g.cell_rot[rb + min(m0 + tid, nr - 1)];
+        srot[tid] = rot;
+        if (MODE == G_LOGL) sbias[tid] = bias[rot];
+    }
+
+    const bf16_t *ga[NA][2];
+    const unsigned char *gb[2];
+    int Kd;
+    if (MODE == G_LOGL) {
+        Kd = Pp;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t arow = g.cell_rot[rb + min(m0 + lrow + 64 * h, nr - 1)];
+            ga[0][h] = A0 + arow * lda + lcol;
+            if (NA > 1) ga[1][h] = A1 + arow * lda + lcol;
+            if (NA > 2) ga[NA - 1][h] = A2 + arow * lda + lcol;
+            const int64_t brow = g.slot_pat[sb + min(n0 + lrow + 64 * h, ns - 1)];
+            gb[h] = reinterpret_cast<const unsigned char *>(Bv) + (brow * ldb + lcol) * (BU8 ? 1 : 2);
+        }
+    } else {
+        Kd = nsp;
+        const int64_t kt0 = (int64_t)g.gbeg[c] * Pp;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t arow = boff + (int64_t)min(m0 + lrow + 64 * h, nr - 1) * nsp + lcol;
+            ga[0][h] = A0 + arow;
+            if (NA > 1) ga[1][h] = A1 + arow;
+            const int64_t brow = kt0 + (int64_t)(n0 + lrow + 64 * h) * nsp + lcol;
+            gb[h] = reinterpret_cast<const unsigned char *>(Bv) + brow * (BU8 ? 1 : 2);
+        }
+    }
+
+    u32x4 ra[NA][2], rb16[2];
+    u32x2 rb8[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) ra[a][h] = *reinterpret_cast<const u32x4 *>(ga[a][h] + k0);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (BU8) rb8[h] = *reinterpret_cast<const u32x2 *>(gb[h] + k0);
+            else rb16[h] = *reinterpret_cast<const u32x4 *>(gb[h] + (size_t)k0 * 2);
+        }
+    };
+    auto sstore = [&](int stage) {
+        bf16_t *S = smem + stage * STAGE;
+#pragma unroll
+        for (int a = 0; a < NA; ++a)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<u32x4 *>(S + ((size_t)a * TM + lrow + 64 * h) * TLD + lcol) = ra[a][h];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            *reinterpret_cast<u32x4 *>(S + ((size_t)NA * TM + lrow + 64 * h) * TLD + lcol) =
+                BU8 ? widen_u8x8(rb8[h]) : rb16[h];
+    };
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+
+    const int nk = Kd / TK;
+    const int fr = lane & 31, fk = (lane >> 5) * 8;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * TK);
+        const bf16_t *S = smem + cur * STAGE;
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 16) {
+            bf16x8 fa[NA][2], fb[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int a = 0; a < NA; ++a)
+                    fa[a][i] = *reinterpret_cast<const bf16x8 *>(S + ((size_t)a * TM + wm + i * 32 + fr) * TLD + kk + fk);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                fb[j] = *reinterpret_cast<const bf16x8 *>(S + ((size_t)NA * TM + wn + j * 32 + fr) * TLD + kk + fk);
+#pragma unroll
+            for (int a = NA - 1; a >= 0; --a)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a][i], fb[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    // accumulator: lane holds column lane % 32 and rows 8*(r/4) + 4*(lane/32) + r%4
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int ml = wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m0 + ml >= nr) continue;
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int n = n0 + wn + j * 32 + (lane & 31);
+                if (MODE == G_LOGL) {
+                    if (n < ns) out[boff + (int64_t)(m0 + ml) * nsp + n] = acc[i][j][r] + sbias[ml];
+                } else {
+                    out[(int64_t)srot[ml] * Pp + n] = acc[i][j][r];
+                }
+            }
+        }
+}
+
+// KTg block of cell c, padded slots g0 .. g0+31: KTg[gbeg_c*Pp + j*nsp_c + g] = K[pat(g)][j]
+// (0 for padding slots).  Loads are row segments of K, stores row segments of KTg.
+template <typename T>
+__global__ __launch_bounds__(256) void k_build_ktg(const T *__restrict__ K, int64_t ldk, int Pp, GTab g,
+                                                   T *__restrict__ ktg)
+{
+    constexpr int PAD = 4 / (int)sizeof(T);
+    __shared__ T tile[32][256 + PAD];
+    const int4 td = g.tiles[blockIdx.x];
+    const int c = td.x, g0 = td.y;
+    const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
+    const int64_t out0 = (int64_t)g.gbeg[c] * Pp;
+    const int tid = threadIdx.x;
+    for (int p0 = 0; p0 < Pp; p0 += 256) {
+        for (int r = 0; r < 32; ++r) {
+            const int s = g0 + r;
+            T v = 0;
+            if (s < ns && p0 + tid < Pp) v = K[(int64_t)g.slot_pat[sb + s] * ldk + p0 + tid];
+            tile[r][tid] = v;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < 256 * 32; idx += 256) {
+            const int p = idx >> 5, s = idx & 31;
+            if (p0 + p < Pp) ktg[out0 + (int64_t)(p0 + p) * nsp + g0 + s] = tile[s][p];
+        }
+        __syncthreads();
+    }
+}
+
+// Softmax partials: one wavefront per 64 slots of a cell (tiles (cell, s0)); online (max, FP64
+// sum) over the cell's rotation rows, as the dense partial pass does over a band.
+__global__ __launch_bounds__(256) void k_gsm_partial(const float *__restrict__ Lb, GTab g, int n_tiles,
+                                                     float *__restrict__ pm, double *__restrict__ ps)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (t >= n_tiles) return;
+    const int4 td = g.tiles[t];
+    const int c = td.x, s = td.y + lane;
+    const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
+    const int nr = g.rbeg[c + 1] - g.rbeg[c];
+    if (s >= ns) return;
+    const float *col = Lb + g.boff[c] + s;
+    float m = -1.0e10f;
+    double sum = 0.0;
+    for (int r = 0; r < nr; ++r) {
+        const float v = col[(int64_t)r * nsp];
+        if (v > m) {
+            sum = sum * (double)expf(m - v) + 1.0;
+            m = v;
+        } else {
+            const float d = v - m;
+            if (d > -1.0e10f) sum += (double)expf(d);
+        }
+    }
+    pm[sb + s] = m;
+    ps[sb + s] = sum;
+}
+
+// Per pattern: combine its slots' partials in FP64 -> lse'_i; per-block sums of lse'.
+__global__ __launch_bounds__(256) void k_gsm_final(const float *__restrict__ pm, const double *__restrict__ ps, GTab g,
+                                                   int I, double *__restrict__ lse, double *__restrict__ blk_sum,
+                                                   double *__restrict__ scal)
+{
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double l = 0.0;
+    if (i < I) {
+        const int q0 = g.pbeg[i], q1 = g.pbeg[i + 1];
+        float m = -1.0e10f;
+        for (int q = q0; q < q1; ++q) m = fmaxf(m, pm[g.pslot[q]]);
+        double sum = 0.0;
+        for (int q = q0; q < q1; ++q) {
+            const int sl = g.pslot[q];
+            const double sv = ps[sl];
+            if (sv > 0.0) sum += sv * exp((double)pm[sl] - (double)m);
+        }
+        l = (double)m + log(sum);
+        if (!isfinite(l)) scal[SC_NONFINITE] = 1.0;
+        lse[i] = l;
+    }
+    l = wave_sum(l);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) blk_sum[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void k_slot_lse(const int32_t *__restrict__ slot_pat, const double *__restrict__ lse, int64_t n,
+                           double *__restrict__ slse)
+{
+    const int64_t s = blockIdx.x * (int64_t)256 + threadIdx.x;
+    if (s < n) slse[s] = lse[slot_pat[s]];
+}
+
+// P = exp(logit - lse) as BF16 hi + lo in the block layout (0 in padding columns) and the slice
+// sums T[rot] (FP64).  One wavefront per fine rotation (in cell order); rotations of cells no
+// pattern selected get T = 0, which the insertion skips.
+__global__ __launch_bounds__(256) void k_gsm_apply(const float *__restrict__ Lb, GTab g, int S,
+                                                   const double *__restrict__ slse, bf16_t *__restrict__ Ph,
+                                                   bf16_t *__restrict__ Pl, double *__restrict__ T)
+{
+    const int gi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (gi >= S) return;
+    const int rot = g.cell_rot[gi], c = g.cell_of[rot], r = gi - g.rbeg[c];
+    const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
+    if (ns == 0) {
+        if (lane == 0) T[rot] = 0.0;
+        return;
+    }
+    const int64_t row = g.boff[c] + (int64_t)r * nsp;
+    double acc = 0.0;
+    for (int s = lane; s < nsp; s += 64) {
+        float p = 0.0f;
+        if (s < ns) p = expf((float)((double)Lb[row + s] - slse[sb + s]));
+        const bf16_t h = f2bf(p);
+        Ph[row + s] = h;
+        Pl[row + s] = f2bf(p - bf2f(h));
+        acc += (double)p;
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) T[rot] = acc;
+}
+
+GTab tables(const emc_ctx *c, const int4 *tiles)
+{
+    const Proposal *p = c->prop;
+    GTab g;
+    g.cell_rot = p->cell_rot;
+    g.rbeg = p->rbeg;
+    g.cell_of = p->cell_of;
+    g.slot_pat = p->lists + p->off_slot_pat;
+    g.sbeg = p->lists + p->off_sbeg;
+    g.nsp = p->lists + p->off_nsp;
+    g.gbeg = p->lists + p->off_gbeg;
+    g.pbeg = p->lists + p->off_pbeg;
+    g.pslot = p->lists + p->off_pslot;
+    g.boff = p->boff;
+    g.tiles = tiles;
+    return g;
+}
+
+const int4 *tile_list(const Proposal *p, int64_t off) { return reinterpret_cast<const int4 *>(p->lists + off); }
+
+}  // namespace
+
+hipError_t launch_grouped_logl(const emc_ctx *c, hipStream_t st)
+{
+    const Proposal *p = c->prop;
+    if (p->n_tiles_e == 0) return hipSuccess;
+    const GTab g = tables(c, tile_list(p, p->off_tile_e));
+    const dim3 grid(p->n_tiles_e), blk(256);
+    const int64_t ld = c->Pp;
+    const bf16_t *A1 = c->la_terms == 3 ? c->Lm : c->Ll;
+    if (c->K8) {
+        if (c->la_terms == 3)
+            hipLaunchKernelGGL((k_grouped<G_LOGL, 3, true>), grid, blk, 0, st, c->Lh, A1, c->Ll, ld, c->K8, ld, c->Pp, g,
+                               c->bias, c->R);
+        else
+            hipLaunchKernelGGL((k_grouped<G_LOGL, 2, true>), grid, blk, 0, st, c->Lh, A1, nullptr, ld, c->K8, ld, c->Pp,
+                               g, c->bias, c->R);
+    } else {
+        if (c->la_terms == 3)
+            hipLaunchKernelGGL((k_grouped<G_LOGL, 3, false>), grid, blk, 0, st, c->Lh, A1, c->Ll, ld, c->K16, ld, c->Pp,
+                               g, c->bias, c->R);
+        else
+            hipLaunchKernelGGL((k_grouped<G_LOGL, 2, false>), grid, blk, 0, st, c->Lh, A1, nullptr, ld, c->K16, ld,
+                               c->Pp, g, c->bias, c->R);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_grouped_softmax(const emc_ctx *c, hipStream_t st)
+{
+    const Proposal *p = c->prop;
+    GTab g = tables(c, tile_list(p, p->off_tile_sm));
+    hipLaunchKernelGGL(k_gsm_partial, dim3((p->n_tiles_sm + 3) / 4), dim3(256), 0, st, c->R, g, p->n_tiles_sm,
+                       p->slot_pm, p->slot_ps);
+    const int ncol = (c->I + 255) / 256;
+    double *blk = c->part;
+    hipLaunchKernelGGL(k_gsm_final, dim3(ncol), dim3(256), 0, st, p->slot_pm, p->slot_ps, g, c->I, c->lse, blk, c->scal);
+    hipError_t e = launch_sum_f64(blk, ncol, c->scal + SC_TOTAL, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_slot_lse, dim3((unsigned)((p->n_slots + 255) / 256)), dim3(256), 0, st, g.slot_pat, c->lse,
+                       p->n_slots, p->slot_lse);
+    hipLaunchKernelGGL(k_gsm_apply, dim3((c->S + 3) / 4), dim3(256), 0, st, c->R, g, c->S, p->slot_lse, c->Ph, c->Pl,
+                       c->T);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_ktg(const emc_ctx *c, hipStream_t st)
+{
+    const Proposal *p = c->prop;
+    if (p->n_tiles_kt == 0) return hipSuccess;
+    const GTab g = tables(c, tile_list(p, p->off_tile_kt));
+    if (c->K8)
+        hipLaunchKernelGGL(k_build_ktg<uint8_t>, dim3(p->n_tiles_kt), dim3(256), 0, st, c->K8, (int64_t)c->Pp, c->Pp, g,
+                           reinterpret_cast<uint8_t *>(p->ktg));
+    else
+        hipLaunchKernelGGL(k_build_ktg<uint16_t>, dim3(p->n_tiles_kt), dim3(256), 0, st, c->K16, (int64_t)c->Pp, c->Pp,
+                           g, reinterpret_cast<uint16_t *>(p->ktg));
+    return hipGetLastError();
+}
+
+hipError_t launch_grouped_mstep(const emc_ctx *c, hipStream_t st)
+{
+    const Proposal *p = c->prop;
+    if (p->n_tiles_m == 0) return hipSuccess;
+    const GTab g = tables(c, tile_list(p, p->off_tile_m));
+    const dim3 grid(p->n_tiles_m), blk(256);
+    if (c->K8)
+        hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, true>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, p->ktg, 0, c->Pp,
+                           g, nullptr, c->U);
+    else
+        hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, false>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, p->ktg, 0, c->Pp,
+                           g, nullptr, c->U);
+    return hipGetLastError();
+}
+
+}  // namespace emc

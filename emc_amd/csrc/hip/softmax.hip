@@ -173,6 +173,12 @@ int64_t softmax_part_len(int S, int I)
            (int64_t)ncb * S + 16;
 }
 
+hipError_t launch_sum_f64(const double *x, int n, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, x, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
 {
     const int S = c->S, I = c->I;

@@ -95,8 +95,13 @@ enum {
     EMC_STAGE_TOTAL = 5,
     EMC_STAGE_LOGL_GEMM = 6,
     EMC_STAGE_MSTEP_GEMM = 7,
-    EMC_N_STAGES = 8
+    EMC_STAGE_PROPOSE = 8, /* proposal: network inference, cell selection, grouped work lists */
+    EMC_STAGE_TRAIN = 9,   /* proposal: cell targets, synthetic patterns, network training */
+    EMC_N_STAGES = 10
 };
+
+/* Search used by an iteration (emc_iter_stats.search). */
+enum { EMC_SEARCH_FULL = 0, EMC_SEARCH_PROPOSAL = 1 };
 
 typedef struct emc_iter_stats {
     double total_logl; /* sum_i lse_i, FP64 (the convergence measure) */
@@ -105,6 +110,14 @@ typedef struct emc_iter_stats {
     double model_avg;  /* mean of positive voxels before normalisation (emc_cuda.cu:639-661) */
     double weight_sum; /* sum of the insertion weights Wt */
     float stage_ms[EMC_N_STAGES]; /* HIP-event times when options.timing = 1 */
+    /* learned proposal (zero / -1 when it is not enabled) */
+    int32_t search;        /* EMC_SEARCH_* */
+    int32_t n_fallback;    /* patterns searched over every cell (predicted mass < min_mass) */
+    double predicted_mass; /* mean over patterns of the network's mass in the kept cells */
+    double captured_mass;  /* full search with a trained network: mean over patterns of the
+                              responsibility mass inside the cells the network would keep; else -1 */
+    double train_loss;     /* mean cross-entropy of the last training pass; -1 if none ran */
+    double work_fraction;  /* scored (rotation, pattern) pairs / (S * I) */
 } emc_iter_stats;
 
 typedef struct emc_ctx emc_ctx;
@@ -209,6 +222,75 @@ emc_status emc_synth_patterns(const float *model, int32_t L, const float *x, con
                               double photons, uint64_t seed, int32_t *counts, float *q_true);
 /* Uniform random [0,1) model (model_input = 0, emc_helper.cpp:449-456, own RNG). */
 emc_status emc_synth_uniform_model(int32_t L, uint64_t seed, float *model);
+
+/* ---- learned orientation proposal (no reference counterpart: main.cpp:239-480 always scores
+ * every rotation for every pattern) -------------------------------------------------------------
+ *
+ * The context's rotations are the fine grid.  A coarse grid (e.g. emc_generate_rotations(2),
+ * 420 rotations) partitions it into cells: fine rotation s belongs to the coarse rotation c with
+ * the largest |q_s . q_c|.  A network  x -> 512 -> 512 -> C  (GELU, softmax over cells; x = log1p
+ * of the pattern pooled onto 32 x 32, standardised) predicts each pattern's responsibility mass
+ * per cell.  It is trained on the device (BF16 MFMA forward/backward, FP32 master weights and
+ * Adam) on the cell sums of the responsibilities that full-search iterations produce, plus
+ * patterns sliced from the current model and Poisson-sampled.  A proposal iteration scores, for
+ * each pattern, only the fine rotations of its top_k cells (all cells when the predicted mass in
+ * them is below min_mass) as a grouped MFMA contraction; its softmax, M-step sum and insertion
+ * see P = 0 elsewhere.  Iterations 0 .. warmup_full-1 and every full_every-th iteration after
+ * them search fully, train the network and measure the captured mass.  Needs an MFMA precision,
+ * TRUE_POISSON, slice_chunk covering all rotations and counts <= 256. */
+typedef struct emc_proposal_options {
+    int32_t top_k;          /* cells kept per pattern (8) */
+    int32_t warmup_full;    /* leading full-search iterations (3) */
+    int32_t full_every;     /* after warm-up, every full_every-th iteration searches fully (5) */
+    int32_t hidden;         /* hidden width, multiple of 64 (512) */
+    int32_t epochs;         /* passes over the training set after each full search (4) */
+    int32_t batch;          /* minibatch rows, multiple of 64 (256) */
+    int32_t synth_patterns; /* model-sliced training patterns per pass; -1 = n_patterns / 4 */
+    int32_t reserved0;
+    float min_mass;         /* per-pattern predicted mass below which it is searched fully (0.99) */
+    float lr;               /* Adam step size (1e-3; beta 0.9 / 0.999, eps 1e-8) */
+    float max_fallback;     /* fraction of fully searched patterns above which the iteration
+                               runs a full search instead (0.1) */
+    float reserved1;
+    uint64_t seed;          /* initial weights, shuffling, synthetic patterns */
+} emc_proposal_options;
+
+void emc_proposal_defaults(emc_proposal_options *o);
+/* Coarse cell of each fine rotation: argmax_c |fine_s . coarse_c| (lowest c on ties).  Host. */
+emc_status emc_coarse_partition(const float *fine, int32_t n_fine, const float *coarse,
+                                int32_t n_coarse, int32_t *cell);
+/* Enable the proposal on a context whose rotations, coords, mask and patterns are set; coarse
+ * [n_coarse][4].  Computes the pattern features, initialises the network (He-uniform) and Adam
+ * state and restarts the iteration counter. */
+emc_status emc_proposal_enable(emc_ctx *ctx, const float *coarse, int32_t n_coarse,
+                               const emc_proposal_options *opt);
+emc_status emc_proposal_disable(emc_ctx *ctx);
+/* Network prediction for every pattern: cells [I][top_k] (ascending), mass [I].  Either may be
+ * NULL. */
+emc_status emc_proposal_predict(emc_ctx *ctx, int32_t *cells, float *mass);
+/* Cell sums of the current full responsibilities: targets [I][n_coarse]. */
+emc_status emc_proposal_cell_targets(emc_ctx *ctx, float *targets);
+/* Train on the current full responsibilities (cell targets) plus synthetic patterns for
+ * `epochs` passes; mean cross-entropy of the last pass in *loss. */
+emc_status emc_proposal_train(emc_ctx *ctx, int32_t epochs, double *loss);
+/* Pooled log1p features of every pattern before standardisation: raw [I][1024]. */
+emc_status emc_proposal_features(emc_ctx *ctx, float *raw);
+/* Network parameters (FP32 masters), layouts W1 [H][1024], b1 [H], W2 [H][H], b2 [H],
+ * W3 [C][H], b3 [C].  set also refreshes the BF16 copies and clears the Adam state. */
+emc_status emc_mlp_get_params(emc_ctx *ctx, float *W1, float *b1, float *W2, float *b2,
+                              float *W3, float *b3);
+emc_status emc_mlp_set_params(emc_ctx *ctx, const float *W1, const float *b1, const float *W2,
+                              const float *b2, const float *W3, const float *b3);
+/* One forward pass on given (already standardised) features x [B][1024] -> probs [B][C]. */
+emc_status emc_mlp_forward(emc_ctx *ctx, const float *x, int32_t B, float *probs);
+/* Gradients of the mean cross-entropy against targets t [B][C] (same layouts as the
+ * parameters); *loss = the loss.  Parameters unchanged. */
+emc_status emc_mlp_gradients(emc_ctx *ctx, const float *x, const float *t, int32_t B, double *loss,
+                             float *gW1, float *gb1, float *gW2, float *gb2, float *gW3,
+                             float *gb3);
+/* One Adam step with the gradients of (x, t) (step counter advances). */
+emc_status emc_mlp_train_step(emc_ctx *ctx, const float *x, const float *t, int32_t B,
+                              double *loss);
 
 /* ---- configuration (emc.conf) ----------------------------------------------------------------- */
 
