@@ -143,6 +143,72 @@ def accuracy(d, n_sub_logl: int, n_model: int):
     return res
 
 
+def run_proposal(name: str, iters: int, steps_note: str = ""):
+    """Learned proposal at 204,960 fine rotations (n=16) / 420 coarse cells (n=2), 50,000
+    patterns: the same iterations from the same start with the full search only (reference) and
+    with the proposal policy; per-iteration times, captured mass, model relative L2 vs full."""
+    import emc_amd
+    from emc_amd import workloads as wl
+    t = time.perf_counter()
+    d = wl.make(name)
+    cq, _ = emc_amd.rotations(d["cfg"]["coarse_n"])
+    S, I, L = d["q"].shape[0], d["K"].shape[0], d["L"]
+    log(f"[proposal] workload {name}: S={S} I={I} coarse={cq.shape[0]} max count {d['K'].max()} "
+        f"(generated in {time.perf_counter() - t:.1f}s)")
+
+    def engine():
+        e = emc_amd.Engine(L, S, I, precision="bf16x2", insert="binned", blur_sigma=d["cfg"]["blur"], timing=1)
+        e.setup(d["q"], d["w"], d["coords"], d["mask"], d["K"], d["model0"])
+        e.normalize_model()
+        return e
+
+    full_models, full_ms = [], []
+    t = time.perf_counter()
+    with engine() as e:
+        log(f"[proposal] full-search engine ready in {time.perf_counter() - t:.1f}s")
+        for it in range(iters):
+            t0 = time.perf_counter()
+            s = e.iterate(it)
+            full_ms.append((time.perf_counter() - t0) * 1e3)
+            full_models.append(e.get_model()[0])
+            log(f"[proposal] full it {it}: {full_ms[-1]:.1f} ms (logL GEMM {s['stage_ms']['logl_gemm']:.1f}, "
+                f"M-step GEMM {s['stage_ms']['mstep_gemm']:.1f})")
+    rows = []
+    t = time.perf_counter()
+    with engine() as e:
+        e.enable_proposal(cq)
+        log(f"[proposal] proposal engine ready in {time.perf_counter() - t:.1f}s")
+        for it in range(iters):
+            t0 = time.perf_counter()
+            s = e.iterate(it)
+            ms = (time.perf_counter() - t0) * 1e3
+            M = e.get_model()[0]
+            rel = float(np.linalg.norm(M - full_models[it]) / np.linalg.norm(full_models[it]))
+            row = {"it": it, "search": s["search"], "ms": ms, "work_fraction": s["work_fraction"],
+                   "n_fallback": s["n_fallback"], "predicted_mass": s["predicted_mass"],
+                   "captured_mass": s["captured_mass"], "train_loss": s["train_loss"],
+                   "model_rel_l2_vs_full": rel,
+                   "stage_ms": {k: round(v, 3) for k, v in s["stage_ms"].items()}}
+            rows.append(row)
+            log(f"[proposal] it {it}: {json.dumps(row)}")
+    prop_ms = [r["ms"] for r in rows if r["search"] == "proposal"]
+    prop_full_ms = [r["ms"] for r in rows if r["search"] == "full" and r["it"] >= 3]
+    caps = [r["captured_mass"] for r in rows if r["captured_mass"] >= 0]
+    ref_ms = float(np.mean(full_ms[1:])) if len(full_ms) > 1 else float(full_ms[0])
+    out = {"workload": name, "fine_rotations": S, "coarse_cells": int(cq.shape[0]), "patterns": I,
+           "iterations": iters, "full_search_ms": ref_ms,
+           "proposal_iteration_ms": float(np.mean(prop_ms)) if prop_ms else None,
+           "speedup_proposal_iteration": ref_ms / float(np.mean(prop_ms)) if prop_ms else None,
+           "full_with_training_ms": float(np.mean(prop_full_ms)) if prop_full_ms else None,
+           "captured_mass": caps, "final_model_rel_l2_vs_full": rows[-1]["model_rel_l2_vs_full"],
+           "per_iteration": rows}
+    if prop_ms and prop_full_ms:
+        # steady state over one period of the policy: (full_every - 1) proposal + 1 full iteration
+        period = 5
+        out["speedup_amortised"] = ref_ms * period / ((period - 1) * float(np.mean(prop_ms)) + float(np.mean(prop_full_ms)))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +222,10 @@ def main():
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
     ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
     ap.add_argument("--accuracy", action="store_true", help="precision against the FP64 oracle (slow)")
+    ap.add_argument("--proposal", action="store_true",
+                    help="also run the learned proposal at 204,960 rotations against the full search")
+    ap.add_argument("--proposal-only", action="store_true", help="only the learned-proposal comparison")
+    ap.add_argument("--proposal-iters", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -167,6 +237,9 @@ def main():
 
     info = emc_amd.device_info(0)
     log(f"[bench] device {info['name']}, {info['n_cu']} CUs, {info['hbm_bytes'] / 2**30:.0f} GiB")
+    if args.proposal_only:
+        print(json.dumps({"proposal": run_proposal("L64_S204960_I50000", args.proposal_iters)}))
+        return
     t = time.perf_counter()
     d = wl.make(args.workload)
     S, I, L = d["q"].shape[0], d["K"].shape[0], d["L"]
@@ -203,6 +276,7 @@ def main():
     acc = accuracy(d, 400, 4000) if args.accuracy else None
     if acc:
         log(f"[bench] accuracy {json.dumps(acc)}")
+    prop = run_proposal("L64_S204960_I50000", args.proposal_iters) if args.proposal else None
 
     cpu = None
     if not args.no_cpu:
@@ -238,6 +312,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": variants or None,
         "accuracy": acc,
+        "proposal": prop,
     }
     print(json.dumps(out))
 
