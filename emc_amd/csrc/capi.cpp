@@ -14,9 +14,6 @@
 
 const char *emc_host_last_error();
 
-namespace emc {
-int64_t softmax_part_len(int S, int I);
-}
 
 using namespace emc;
 
@@ -300,7 +297,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (st == EMC_OK) A(dalloc(&c->cst, Ip), "cst");
     if (st == EMC_OK) A(dalloc(&c->R, Sp * Ip), "R");
     if (st == EMC_OK) A(dalloc(&c->lse, Ip), "lse");
-    c->part_len = std::max<int64_t>(emc::softmax_part_len(c->S, c->I), 4096);
+    c->part_len = std::max<int64_t>(emc::softmax_part_len(c->S, c->I, softmax_fused_bands(c)), 4096);
     if (st == EMC_OK) A(dalloc(&c->part, c->part_len), "part");
     if (st == EMC_OK) A(dalloc(&c->T, Sp), "T");
     if (st == EMC_OK && !(c->mfma && opt.insert == EMC_INSERT_FUSED)) A(dalloc(&c->U, cp * PpMax), "U");

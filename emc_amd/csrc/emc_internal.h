@@ -199,7 +199,11 @@ hipError_t launch_mstep_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st);
 hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipStream_t st);
 
-// Responsibilities.
+// Responsibilities.  When logl_fuses_softmax(c), the logL contraction's epilogue writes the
+// per-band softmax partials (one band per 256 rotation rows) and launch_softmax skips its pass.
+bool logl_fuses_softmax(const emc_ctx *c);
+int softmax_fused_bands(const emc_ctx *c);
+int64_t softmax_part_len(int S, int I, int fused_bands);
 hipError_t launch_softmax(const emc_ctx *c, hipStream_t st);
 // Atomic insertion of U rows [s0, s0+cs).
 hipError_t launch_insert(const emc_ctx *c, int s0, int cs, hipStream_t st);

@@ -57,40 +57,44 @@ __global__ __launch_bounds__(256) void k_blur_axis(const float *__restrict__ in,
 }
 
 // Per-block FP64 sum and count of positive voxels and the max (emc_cuda.cu:629-661).
-__global__ __launch_bounds__(256) void k_model_partial(const float *__restrict__ M, int64_t n,
-                                                       double *__restrict__ part)
+__global__ __launch_bounds__(256) void k_model_partial(const float *__restrict__ M, const float *__restrict__ Wt,
+                                                       int64_t n, double *__restrict__ part)
 {
-    __shared__ double rs[4], rc[4], rm[4];
-    double s = 0.0, cnt = 0.0, mx = -1e300;
+    __shared__ double rs[4], rc[4], rm[4], rw[4];
+    double s = 0.0, cnt = 0.0, mx = -1e300, ws = 0.0;
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const float x = M[v];
         if (x > 0.0f) { s += x; cnt += 1.0; }
         mx = fmax(mx, (double)x);
+        if (Wt) ws += Wt[v];
     }
     s = wave_sum(s);
     cnt = wave_sum(cnt);
+    ws = wave_sum(ws);
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
-    if ((threadIdx.x & 63) == 0) { rs[threadIdx.x >> 6] = s; rc[threadIdx.x >> 6] = cnt; rm[threadIdx.x >> 6] = mx; }
+    if ((threadIdx.x & 63) == 0) {
+        rs[threadIdx.x >> 6] = s; rc[threadIdx.x >> 6] = cnt; rm[threadIdx.x >> 6] = mx; rw[threadIdx.x >> 6] = ws;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        part[3 * blockIdx.x + 0] = rs[0] + rs[1] + rs[2] + rs[3];
-        part[3 * blockIdx.x + 1] = rc[0] + rc[1] + rc[2] + rc[3];
-        part[3 * blockIdx.x + 2] = fmax(fmax(rm[0], rm[1]), fmax(rm[2], rm[3]));
+        part[4 * blockIdx.x + 0] = rs[0] + rs[1] + rs[2] + rs[3];
+        part[4 * blockIdx.x + 1] = rc[0] + rc[1] + rc[2] + rc[3];
+        part[4 * blockIdx.x + 2] = fmax(fmax(rm[0], rm[1]), fmax(rm[2], rm[3]));
+        part[4 * blockIdx.x + 3] = rw[0] + rw[1] + rw[2] + rw[3];
     }
 }
 
 __global__ __launch_bounds__(256) void k_model_final(const double *__restrict__ part, int nb, double *__restrict__ scal,
-                                                     const float *__restrict__ Wt, int64_t n)
+                                                     bool with_wt)
 {
     __shared__ double rs[4], rc[4], rm[4], rw[4];
     double s = 0.0, cnt = 0.0, mx = -1e300, ws = 0.0;
     for (int b = threadIdx.x; b < nb; b += 256) {
-        s += part[3 * b];
-        cnt += part[3 * b + 1];
-        mx = fmax(mx, part[3 * b + 2]);
+        s += part[4 * b];
+        cnt += part[4 * b + 1];
+        mx = fmax(mx, part[4 * b + 2]);
+        ws += part[4 * b + 3];
     }
-    if (Wt)
-        for (int64_t v = threadIdx.x; v < n; v += 256) ws += Wt[v];
     s = wave_sum(s);
     cnt = wave_sum(cnt);
     ws = wave_sum(ws);
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(256) void k_model_final(const double *__restrict__ 
         scal[SC_COUNT] = C;
         scal[SC_INV] = avg > 0.0 ? 1.0 / avg : 1.0;
         scal[SC_MAX] = fmax(fmax(rm[0], rm[1]), fmax(rm[2], rm[3]));
-        if (Wt) scal[SC_WSUM] = rw[0] + rw[1] + rw[2] + rw[3];
+        if (with_wt) scal[SC_WSUM] = rw[0] + rw[1] + rw[2] + rw[3];
         if (!(avg > 0.0) || !isfinite(avg)) scal[SC_NONFINITE] = 2.0;
     }
 }
@@ -143,9 +147,9 @@ hipError_t launch_blur(const emc_ctx *c, hipStream_t st)
 
 hipError_t launch_normalize(const emc_ctx *c, bool apply, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_model_partial, dim3(kPartBlocks), dim3(256), 0, st, c->model, c->N3, c->part);
-    hipLaunchKernelGGL(k_model_final, dim3(1), dim3(256), 0, st, c->part, kPartBlocks, c->scal,
-                       apply ? c->Wt : nullptr, c->N3);
+    hipLaunchKernelGGL(k_model_partial, dim3(kPartBlocks), dim3(256), 0, st, c->model, apply ? c->Wt : nullptr, c->N3,
+                       c->part);
+    hipLaunchKernelGGL(k_model_final, dim3(1), dim3(256), 0, st, c->part, kPartBlocks, c->scal, apply);
     if (apply)
         hipLaunchKernelGGL(k_scale, dim3((unsigned)((c->N3 + 255) / 256)), dim3(256), 0, st, c->model, c->N3,
                            c->scal);

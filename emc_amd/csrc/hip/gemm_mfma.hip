@@ -40,6 +40,11 @@ struct EpiArgs {
     const float *axyz;   // [3][Pp]
     int Pp, Pa, L, interp;
     float *Mnum, *Wt;
+    // softmax partials of the logL epilogue (large-tile kernel): per 256-row band and pattern,
+    // the band maximum pm and sum of exp(R - pm) ps; band = band0 + m-tile
+    float *pm;
+    double *ps;
+    int64_t band0, ldp;
 };
 
 __device__ __forceinline__ void scatter_one(const EpiArgs &e, const float *m, float Ts, int j, float u)
@@ -248,7 +253,7 @@ template <int NA, int EPI, int BN2, bool BU8>
 __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
                                                     const bf16_t *__restrict__ A2, int64_t lda,
                                                     const void *__restrict__ Bv, int64_t ldb, int Kd, int ntn,
-                                                    EpiArgs e)
+                                                    int n_base, EpiArgs e)
 {
     constexpr int TN = BN2 / 64;                       // 32-wide tiles per wave along N
     constexpr int BCH = BN2 * (BK2 / 8) / 512;         // 16-byte B chunks per thread per stage
@@ -259,7 +264,7 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
     const int nwg = gridDim.x, b = blockIdx.x;
     const int base = nwg / 8, rem = nwg % 8, xcd = b % 8, loc = b / 8;
     const int t = xcd * base + min(xcd, rem) + loc;
-    const int m0 = (t / ntn) * BM2, n0 = (t % ntn) * BN2;
+    const int m0 = (t / ntn) * BM2, n0 = n_base + (t % ntn) * BN2;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN2 / 2);
@@ -306,15 +311,9 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
 
-    const int nk = Kd / BK2;
     const int fr = lane & 31, fk = (lane >> 5) * 8;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload((kt + 1) * BK2);
-        const bf16_t *S = smem + cur * STAGE;
+    auto compute = [&](int stage) {
+        const bf16_t *S = smem + stage * STAGE;
 #pragma unroll
         for (int kk = 0; kk < BK2; kk += 16) {
             bf16x8 fa[NA][2], fb[TN];
@@ -334,10 +333,26 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
                     for (int j = 0; j < TN; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[a][i], fb[j], acc[i][j], 0, 0, 0);
         }
+    };
+
+    // the global loads of step k+1 are in flight while step k is multiplied
+    const int nk = Kd / BK2;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload((kt + 1) * BK2);
+        compute(cur);
         if (kt + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
 
+    // accumulator layout: lane holds column lane % 32 of each 32x32 tile and rows
+    // 8*(r/4) + 4*(lane/32) + r%4, r = 0..15; the stored value is acc + bias (logL) or acc
+    float cmax[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) cmax[j] = -1.0e10f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -348,38 +363,101 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const int n = n0 + wn + j * 32 + (lane & 31);
-                if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = acc[i][j][r] + add;
+                const float v = acc[i][j][r] + add;
+                if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = v;
+                if (EPI == EPI_LOGL) cmax[j] = fmaxf(cmax[j], v);
             }
         }
+    if (EPI != EPI_LOGL || !e.pm) return;
+
+    // Softmax partials of this 256-row band (restates k_sm_partial's online max / sum of exp
+    // with the same -1e10 floors): column maxima over the lane pair, then over the 4 row-waves
+    // through LDS; then sums of exp(v - max) the same way.  The main loop's LDS is free.
+    float *red = reinterpret_cast<float *>(smem);            // [4][BN2] maxima
+    double *reds = reinterpret_cast<double *>(smem) + 2 * BN2;  // [4][BN2] sums (after the maxima)
+    const int wrow = wave >> 1;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        cmax[j] = fmaxf(cmax[j], __shfl_xor(cmax[j], 32, 64));
+        if (lane < 32) red[wrow * BN2 + wn + j * 32 + lane] = cmax[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int cl = wn + j * 32 + (lane & 31);
+        cmax[j] = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
+    }
+    double csum[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) csum[j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m >= e.m_valid) continue;
+            const float add = e.bias[m];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const float d = (acc[i][j][r] + add) - cmax[j];
+                if (d > -1.0e10f) csum[j] += (double)expf(d);
+            }
+        }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        csum[j] += __shfl_xor(csum[j], 32, 64);
+        if (lane < 32) reds[wrow * BN2 + wn + j * 32 + lane] = csum[j];
+    }
+    __syncthreads();
+    for (int cl = tid; cl < BN2; cl += 512) {
+        const int n = n0 + cl;
+        if (n >= e.n_valid) continue;
+        const float mx = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
+        const double sm = reds[cl] + reds[BN2 + cl] + reds[2 * BN2 + cl] + reds[3 * BN2 + cl];
+        const int64_t o = (e.band0 + m0 / BM2) * e.ldp + n;
+        e.pm[o] = mx;
+        e.ps[o] = sm;
+    }
 }
 
 template <int NA, int EPI, int BN2, bool BU8>
-hipError_t run2(int ntm, int ntn, hipStream_t st, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2, int64_t lda,
-                const void *B, int64_t ldb, int Kd, const EpiArgs &e)
+hipError_t run2(int ntm, int ntn, int n_base, hipStream_t st, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
+                int64_t lda, const void *B, int64_t ldb, int Kd, const EpiArgs &e)
 {
     hipLaunchKernelGGL((k_gemm_mfma2<NA, EPI, BN2, BU8>), dim3(ntm * ntn), dim3(512), 0, st, A0, A1, A2, lda, B, ldb,
-                       Kd, ntn, e);
+                       Kd, ntn, n_base, e);
     return hipGetLastError();
 }
 
-// Large-tile kernel for a single-term B operand (BF16, or bytes when B8 is given): BN2 = 256 when
-// N allows it and A has 2 terms.
+// Large-tile kernel for a single-term B operand (BF16, or bytes when B8 is given).  Two A terms:
+// 256-wide column tiles, plus one 128-wide launch for the remainder when N % 256 = 128; three A
+// terms: 128-wide tiles (LDS).
+template <int EPI, bool U8>
+hipError_t run_large_t(int M, int N, hipStream_t st, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
+                       int64_t lda, const void *B, int64_t ldb, int Kd, const EpiArgs &e)
+{
+    const int ntm = M / BM2;
+    if (na == 3) return run2<3, EPI, 128, U8>(ntm, N / 128, 0, st, A0, A1, A2, lda, B, ldb, Kd, e);
+    const int n256 = N / 256;
+    hipError_t err = hipSuccess;
+    if (n256 > 0) err = run2<2, EPI, 256, U8>(ntm, n256, 0, st, A0, A1, A2, lda, B, ldb, Kd, e);
+    if (err == hipSuccess && N % 256) err = run2<2, EPI, 128, U8>(ntm, 1, n256 * 256, st, A0, A1, A2, lda, B, ldb, Kd, e);
+    return err;
+}
+
 template <int EPI>
 hipError_t run_large(int M, int N, hipStream_t st, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
                      int64_t lda, const bf16_t *B16, const uint8_t *B8, int64_t ldb, int Kd, const EpiArgs &e)
 {
-    const int ntm = M / BM2;
-    if (B8) {
-        if (na == 2 && N % 256 == 0) return run2<2, EPI, 256, true>(ntm, N / 256, st, A0, A1, A2, lda, B8, ldb, Kd, e);
-        if (na == 2) return run2<2, EPI, 128, true>(ntm, N / 128, st, A0, A1, A2, lda, B8, ldb, Kd, e);
-        return run2<3, EPI, 128, true>(ntm, N / 128, st, A0, A1, A2, lda, B8, ldb, Kd, e);
-    }
-    if (na == 2 && N % 256 == 0) return run2<2, EPI, 256, false>(ntm, N / 256, st, A0, A1, A2, lda, B16, ldb, Kd, e);
-    if (na == 2) return run2<2, EPI, 128, false>(ntm, N / 128, st, A0, A1, A2, lda, B16, ldb, Kd, e);
-    return run2<3, EPI, 128, false>(ntm, N / 128, st, A0, A1, A2, lda, B16, ldb, Kd, e);
+    if (B8) return run_large_t<EPI, true>(M, N, st, na, A0, A1, A2, lda, B8, ldb, Kd, e);
+    return run_large_t<EPI, false>(M, N, st, na, A0, A1, A2, lda, B16, ldb, Kd, e);
 }
 
 }  // namespace
+
+bool logl_fuses_softmax(const emc_ctx *c) { return c->mfma && !c->k_split && c->opt.mfma_tile == 0; }
+
+int softmax_fused_bands(const emc_ctx *c) { return (int)((int64_t)((c->S + c->chunk - 1) / c->chunk) * (c->chunkp / BM2)); }
 
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
 {
@@ -392,9 +470,16 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
     const dim3 grid(c->Ip / BN, c->chunkp / BM);
     const bf16_t *A1 = c->la_terms == 3 ? c->Lm : c->Ll;
     const bf16_t *A2 = c->Ll;
-    if (!c->k_split && c->opt.mfma_tile == 0)
+    if (logl_fuses_softmax(c)) {
+        // band partials in c->part: pm (floats) [bands][I] then ps (doubles) [bands][I]
+        const int64_t nb = softmax_fused_bands(c);
+        e.pm = reinterpret_cast<float *>(c->part);
+        e.ps = c->part + nb * c->I;
+        e.band0 = (int64_t)(s0 / c->chunk) * (c->chunkp / BM2);
+        e.ldp = c->I;
         return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->K8, c->Pp,
                                    c->Pp, e);
+    }
     if (c->la_terms == 3) {
         if (c->k_split) return run<3, 2, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, c->K16lo, c->Pp, c->Pp, e);
         return run<3, 1, EPI_LOGL>(grid, st, c->Lh, A1, A2, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);

@@ -165,11 +165,17 @@ __global__ void k_T_final(const double *__restrict__ Tpart, int ncb, int S, int 
 }  // namespace
 
 // Partials layout inside c->part:  pm (float, nb*I) | ps (double, nb*I) | blk (double) | Tpart.
-int64_t softmax_part_len(int S, int I)
+// nb = the partial pass's band count, or (MFMA logL with fused partials) one band per 256 rows.
+int softmax_bands(int S, int I)
 {
-    const int nb = std::max(1, std::min((S + 63) / 64, 4096 / std::max(1, (I + 255) / 256) + 1));
+    return std::max(1, std::min((S + 63) / 64, 4096 / std::max(1, (I + 255) / 256) + 1));
+}
+
+int64_t softmax_part_len(int S, int I, int fused_bands)
+{
+    const int64_t nb = std::max(softmax_bands(S, I), fused_bands);
     const int ncb = (I + kApplyCols - 1) / kApplyCols;
-    return (int64_t)nb * I /*pm as floats packed in doubles: over-allocate*/ + (int64_t)nb * I + (I + 255) / 256 + 1 +
+    return nb * I /*pm as floats packed in doubles: over-allocate*/ + nb * I + (I + 255) / 256 + 1 +
            (int64_t)ncb * S + 16;
 }
 
@@ -183,14 +189,16 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
 {
     const int S = c->S, I = c->I;
     const int ncol = (I + 255) / 256;
-    const int nb = std::max(1, std::min((S + 63) / 64, 4096 / std::max(1, ncol) + 1));
+    const bool fused = logl_fuses_softmax(c);  // the logL epilogue wrote the band partials
+    const int nb = fused ? softmax_fused_bands(c) : softmax_bands(S, I);
     const int rows = (S + nb - 1) / nb;
     double *base = c->part;
     float *pm = reinterpret_cast<float *>(base);
     double *ps = base + (int64_t)nb * I;
     double *blk = ps + (int64_t)nb * I;
     double *Tpart = blk + ncol + 1;
-    hipLaunchKernelGGL(k_sm_partial, dim3(ncol, nb), dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, rows, pm, ps);
+    if (!fused)
+        hipLaunchKernelGGL(k_sm_partial, dim3(ncol, nb), dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, rows, pm, ps);
     hipLaunchKernelGGL(k_sm_final, dim3(ncol), dim3(256), 0, st, pm, ps, nb, I, c->lse, blk, c->scal);
     hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, blk, ncol, c->scal + SC_TOTAL);
     const int ncb = (I + kApplyCols - 1) / kApplyCols;
