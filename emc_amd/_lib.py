@@ -65,7 +65,7 @@ class Config(C.Structure):
                [(n, C.c_int32) for n in ("precision", "interp", "insert", "flat_detector")] + \
                [("fluence", C.c_double), ("synth_atoms", C.c_int32)] + \
                [(n, C.c_double) for n in ("synth_radius", "synth_photons", "beamstop", "outer_radius")] + \
-               [("synth_seed", C.c_uint64)]
+               [("synth_seed", C.c_uint64), ("proposal_coarse_n", C.c_int32), ("proposal_top_k", C.c_int32)]
 
 
 class EmcError(RuntimeError):

@@ -99,7 +99,18 @@ int main(int argc, char **argv)
     if (guard(emc_set_rotations(ctx, q.data(), w.data()), "set_rotations") &&
         guard(emc_set_coords(ctx, x.data(), y.data(), z.data()), "set_coords") &&
         guard(emc_set_mask(ctx, mask.data()), "set_mask") && guard(emc_set_patterns_i32(ctx, K.data()), "set_patterns") &&
-        guard(emc_set_model(ctx, model.data()), "set_model") && guard(emc_normalize_model(ctx, nullptr), "normalize")) {
+        guard(emc_set_model(ctx, model.data()), "set_model") && guard(emc_normalize_model(ctx, nullptr), "normalize") &&
+        (cfg.proposal_coarse_n == 0 || [&] {
+            // learned proposal over the cells of the coarse grid
+            const int Sc = emc_rotation_count(cfg.proposal_coarse_n);
+            std::vector<float> cq((size_t)4 * Sc), cw(Sc);
+            if (!guard(emc_generate_rotations(cfg.proposal_coarse_n, cq.data(), cw.data()), "coarse rotations"))
+                return false;
+            emc_proposal_options po;
+            emc_proposal_defaults(&po);
+            po.top_k = cfg.proposal_top_k;
+            return guard(emc_proposal_enable(ctx, cq.data(), Sc, &po), "proposal_enable");
+        }())) {
         FILE *tf = std::fopen("ExeTime.data", "wb");
         if (cfg.isOutput) mkdir("output", 0755);
         for (int it = 0; it < cfg.max_iterations && rc == 0; ++it) {
@@ -108,11 +119,14 @@ int main(int argc, char **argv)
             if (!guard(emc_iterate(ctx, it, &s), "iterate")) break;
             const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             std::printf("{\"iteration\": %d, \"sigma\": %.6g, \"total_logl\": %.10g, \"fluence\": %.6g, "
-                        "\"model_avg\": %.6g, \"seconds\": %.6f, \"stage_ms\": {\"slice\": %.3f, \"logl\": %.3f, "
-                        "\"softmax\": %.3f, \"mstep\": %.3f, \"compress\": %.3f}}\n",
-                        it, s.sigma, s.total_logl, s.fluence, s.model_avg, sec, s.stage_ms[EMC_STAGE_SLICE],
-                        s.stage_ms[EMC_STAGE_LOGL], s.stage_ms[EMC_STAGE_SOFTMAX], s.stage_ms[EMC_STAGE_MSTEP],
-                        s.stage_ms[EMC_STAGE_COMPRESS]);
+                        "\"model_avg\": %.6g, \"seconds\": %.6f, \"search\": \"%s\", \"work_fraction\": %.6g, "
+                        "\"captured_mass\": %.6g, \"stage_ms\": {\"slice\": %.3f, \"logl\": %.3f, "
+                        "\"softmax\": %.3f, \"mstep\": %.3f, \"compress\": %.3f, \"propose\": %.3f, \"train\": %.3f}}\n",
+                        it, s.sigma, s.total_logl, s.fluence, s.model_avg, sec,
+                        s.search == EMC_SEARCH_PROPOSAL ? "proposal" : "full", s.work_fraction, s.captured_mass,
+                        s.stage_ms[EMC_STAGE_SLICE], s.stage_ms[EMC_STAGE_LOGL], s.stage_ms[EMC_STAGE_SOFTMAX],
+                        s.stage_ms[EMC_STAGE_MSTEP], s.stage_ms[EMC_STAGE_COMPRESS], s.stage_ms[EMC_STAGE_PROPOSE],
+                        s.stage_ms[EMC_STAGE_TRAIN]);
             std::fflush(stdout);
             if (tf) {
                 std::fprintf(tf, "%d %d   %e \n", it, 0, sec);

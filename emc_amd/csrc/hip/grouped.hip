@@ -206,29 +206,46 @@ __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, 
 }
 
 // KTg block of cell c, padded slots g0 .. g0+31: KTg[gbeg_c*Pp + j*nsp_c + g] = K[pat(g)][j]
-// (0 for padding slots).  Loads are row segments of K, stores row segments of KTg.
+// (0 for padding slots).  Per 128-pixel chunk: 16-byte loads of the 32 pattern rows into LDS,
+// then each pixel's 32 slot values leave as 16-byte stores (two threads per pixel).
 template <typename T>
 __global__ __launch_bounds__(256) void k_build_ktg(const T *__restrict__ K, int64_t ldk, int Pp, GTab g,
                                                    T *__restrict__ ktg)
 {
-    constexpr int PAD = 4 / (int)sizeof(T);
-    __shared__ T tile[32][256 + PAD];
+    constexpr int ES = (int)sizeof(T);
+    constexpr int PW = 128;                 // pixels per chunk
+    constexpr int RC = PW * ES / 16;        // 16-byte chunks per row segment
+    constexpr int LDT = PW + 16 / ES;       // LDS row pitch (elements), 16-byte aligned
+    __shared__ __attribute__((aligned(16))) T tile[32][LDT];
     const int4 td = g.tiles[blockIdx.x];
     const int c = td.x, g0 = td.y;
     const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
     const int64_t out0 = (int64_t)g.gbeg[c] * Pp;
     const int tid = threadIdx.x;
-    for (int p0 = 0; p0 < Pp; p0 += 256) {
-        for (int r = 0; r < 32; ++r) {
-            const int s = g0 + r;
-            T v = 0;
-            if (s < ns && p0 + tid < Pp) v = K[(int64_t)g.slot_pat[sb + s] * ldk + p0 + tid];
-            tile[r][tid] = v;
+    const int p = tid >> 1, half = tid & 1;  // output: pixel p of the chunk, slots 16*half .. +15
+    for (int p0 = 0; p0 < Pp; p0 += PW) {    // Pp is a multiple of 128
+        for (int q = tid; q < 32 * RC; q += 256) {
+            const int r = q / RC, cc = q % RC, s = g0 + r;
+            u32x4 v = {0u, 0u, 0u, 0u};
+            if (s < ns)
+                v = *reinterpret_cast<const u32x4 *>(K + (int64_t)g.slot_pat[sb + s] * ldk + p0 + cc * (16 / ES));
+            *reinterpret_cast<u32x4 *>(&tile[r][cc * (16 / ES)]) = v;
         }
         __syncthreads();
-        for (int idx = tid; idx < 256 * 32; idx += 256) {
-            const int p = idx >> 5, s = idx & 31;
-            if (p0 + p < Pp) ktg[out0 + (int64_t)(p0 + p) * nsp + g0 + s] = tile[s][p];
+        T *dst = ktg + out0 + (int64_t)(p0 + p) * nsp + g0 + 16 * half;
+        constexpr int PER = 4 / ES;  // values per 32-bit word
+#pragma unroll
+        for (int k = 0; k < ES; ++k) {  // 16 values = ES stores of 16 bytes
+            u32x4 o;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int b = 0; b < PER; ++b)
+                    word |= (uint32_t)tile[16 * half + k * 4 * PER + w * PER + b][p] << (8 * ES * b);
+                o[w] = word;
+            }
+            reinterpret_cast<u32x4 *>(dst)[k] = o;
         }
         __syncthreads();
     }

@@ -176,6 +176,8 @@ extern "C" emc_status emc_config_defaults(emc_config *c)
     c->synth_seed = 12;
     c->beamstop = 4.0;
     c->outer_radius = 32.0;
+    c->proposal_coarse_n = 0;
+    c->proposal_top_k = 8;
     return EMC_OK;
 }
 
@@ -298,12 +300,17 @@ extern "C" emc_status emc_config_parse(const char *text, emc_config *c)
             out.synth_seed = (uint64_t)(uint32_t)s;
         } else if (key == "beamstop") ok = as_float(out.beamstop);
         else if (key == "outer_radius") ok = as_float(out.outer_radius);
+        else if (key == "proposal_coarse_n") ok = as_int(out.proposal_coarse_n);
+        else if (key == "proposal_top_k") ok = as_int(out.proposal_top_k);
         else return emc_host_fail(EMC_E_INVALID, (where + ": unknown setting").c_str());
         if (!ok) return emc_host_fail(EMC_E_INVALID, (where + ": wrong value type").c_str());
     }
     (void)diff_seen;
     if (out.model_side < 4 || out.rotations_n < 1 || out.N_images < 1 || out.max_iterations < 0 || out.slice_chunk < 0)
         return emc_host_fail(EMC_E_INVALID, "config: model_side, rotations_n, N_images, max_iterations out of range");
+    if (out.proposal_coarse_n < 0 || out.proposal_coarse_n > 64 || out.proposal_top_k < 1 || out.proposal_top_k > 64)
+        return emc_host_fail(EMC_E_INVALID, "config: proposal_coarse_n or proposal_top_k out of range "
+                                            "([0, 64] and [1, 64])");
     *c = out;
     return EMC_OK;
 }

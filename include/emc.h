@@ -314,6 +314,9 @@ typedef struct emc_config {
     int32_t synth_atoms;
     double synth_radius, synth_photons, beamstop, outer_radius;
     uint64_t synth_seed;
+    /* learned proposal: coarse grid sampling n (0 = off; 2 gives 420 cells) and cells kept per
+     * pattern (emc_proposal_options.top_k); the other proposal options keep their defaults */
+    int32_t proposal_coarse_n, proposal_top_k;
 } emc_config;
 
 /* Defaults for every key (the reference leaves missing keys uninitialised). */

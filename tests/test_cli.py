@@ -53,3 +53,19 @@ def test_cli_synthetic_run(tmp_path):
     assert m.size == 24 ** 3 and w.size == 24 ** 3
     assert abs(m[m > 0].mean() - 1.0) < 1e-4   # normalised to mean 1 over positive voxels
     assert np.all((w > 0) | (m == -1) | (m < 0))
+    assert all(l["search"] == "full" for l in lines)
+
+
+@pytest.mark.gpu
+def test_cli_with_proposal(tmp_path):
+    """proposal_coarse_n = 1 (60 cells): warm-up full searches, then proposal iterations."""
+    print("[cli] synthetic 24^3 run with the learned proposal", flush=True)
+    p = tmp_path / "emc.conf"
+    p.write_text(CONF.replace("max_iterations = 2;", "max_iterations = 5;") +
+                 "proposal_coarse_n = 1; proposal_top_k = 8;\n")
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    assert [l["search"] for l in lines][:3] == ["full"] * 3
+    assert all(np.isfinite(l["total_logl"]) for l in lines)
+    assert lines[3]["search"] in ("proposal", "full")  # full when too many patterns fall back

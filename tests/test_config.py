@@ -33,6 +33,7 @@ isDebug = 0;
 /* engine keys */
 precision = "fp32"; interp : "nearest"; insert = "atomic";
 fluence = 2;
+proposal_coarse_n = 2; proposal_top_k = 6;
 """
 
 
@@ -46,6 +47,7 @@ def test_parse_reference_keys_and_engine_keys():
     assert c["normalize_images"] == 0 and c["known_intensity"] == 1
     assert (c["output_period"], c["output_loop"], c["isOutput"], c["isDebug"]) == (3, 1, 1, 0)
     assert (c["precision"], c["interp"], c["insert"], c["fluence"]) == (0, 0, 0, 2.0)
+    assert (c["proposal_coarse_n"], c["proposal_top_k"]) == (2, 6)
 
 
 @pytest.mark.parametrize("kind,code", [("absolute", 0), ("poisson", 1), ("relative", 2), ("true_poisson", 3)])
@@ -62,7 +64,7 @@ def test_defaults_for_missing_keys():
 
 @pytest.mark.parametrize("bad", ['model_sidee = 64;', 'model_side = 64.5;', 'diff_type = "chi";',
                                  'model_side 64;', 'image_prefix = "unterminated;', 'precision = "fp16";',
-                                 'model_side = 2;'])
+                                 'model_side = 2;', 'proposal_top_k = 0;', 'proposal_coarse_n = -1;'])
 def test_errors_are_reported(bad):
     with pytest.raises(emc_amd.EmcError) as e:
         config.parse(bad)
