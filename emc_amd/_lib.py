@@ -47,7 +47,7 @@ class IterStats(C.Structure):
 
 class ProposalOptions(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("top_k", "warmup_full", "full_every", "hidden", "epochs", "batch",
-                                          "synth_patterns", "reserved0")] + \
+                                          "synth_patterns", "refresh_epochs")] + \
                [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "reserved1")] + \
                [("seed", C.c_uint64)]
 

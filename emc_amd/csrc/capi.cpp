@@ -880,18 +880,22 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         EMC_TRY(logl_enqueue(c, (float)sigma));
         EMC_TRY(softmax_enqueue(c));
         EMC_TRY(mstep_enqueue(c));
-        if (pr) {
-            ev_start(c, EMC_STAGE_TRAIN);
-            EMC_TRY(proposal_targets(c, predicted, &captured));
-            EMC_TRY(proposal_train(c, pr->o.epochs, &train_loss));
-            ev_stop(c, EMC_STAGE_TRAIN);
-        }
+        if (pr) EMC_TRY(proposal_targets(c, predicted, &captured));  // cell sums of the full P
     } else {
         EMC_TRY(sparse_logl_enqueue(c, (float)sigma));
         EMC_TRY(sparse_softmax_enqueue(c));
         EMC_TRY(sparse_mstep_enqueue(c));
     }
     EMC_TRY(compress_enqueue(c));
+    if (pr) {
+        // after compress, so the model-sliced training patterns come from the model the next
+        // iteration uses: full iterations train on the targets plus those, proposal iterations
+        // refresh on those alone
+        ev_start(c, EMC_STAGE_TRAIN);
+        if (full) EMC_TRY(proposal_train(c, pr->o.epochs, false, &train_loss));
+        else if (pr->o.refresh_epochs > 0) EMC_TRY(proposal_train(c, pr->o.refresh_epochs, true, &train_loss));
+        ev_stop(c, EMC_STAGE_TRAIN);
+    }
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL + 1], c->stream);
     EMC_TRY(sync_scalars(c));
     if (pr) ++pr->iter;
@@ -926,7 +930,7 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
                 if (hipEventElapsedTime(&ms, c->ev[2 * EMC_STAGE_PROPOSE], c->ev[2 * EMC_STAGE_PROPOSE + 1]) == hipSuccess)
                     stats->stage_ms[EMC_STAGE_PROPOSE] = ms;
             }
-            if (pr && full) {
+            if (pr) {
                 float ms = 0.0f;
                 if (hipEventElapsedTime(&ms, c->ev[2 * EMC_STAGE_TRAIN], c->ev[2 * EMC_STAGE_TRAIN + 1]) == hipSuccess)
                     stats->stage_ms[EMC_STAGE_TRAIN] = ms;

@@ -256,8 +256,9 @@ emc_status proposal_predict(emc_ctx *c);
 emc_status proposal_build_lists(emc_ctx *c, bool *too_many);
 // Cell targets from the dense responsibilities; captured mass of the last prediction if asked.
 emc_status proposal_targets(emc_ctx *c, bool measure, double *captured);
-// Training passes over the targets plus synthetic patterns; loss of the last pass.
-emc_status proposal_train(emc_ctx *c, int epochs, double *loss);
+// Training passes over the targets plus synthetic patterns (or synthetic patterns only, sliced
+// from the current model); loss of the last pass.
+emc_status proposal_train(emc_ctx *c, int epochs, bool synth_only, double *loss);
 
 // BF16 helpers (host and device agree: round to nearest even, finite inputs).
 __host__ __device__ inline bf16_t f2bf(float x)

@@ -384,29 +384,39 @@ __device__ float poisson_dev(float lam, DevRng &r)
 }
 
 // Synthetic training patterns: slice of the current model at rotation syn_rot[b], scaled so its
-// expected total over the active pixels is `photons`, Poisson-sampled.
+// expected total over the active pixels is `photons`, Poisson-sampled.  Pass 1 writes the slice
+// values and sums them (FP64); pass 2 samples in place.
 template <int INTERP>
 __global__ __launch_bounds__(256) void k_synth_counts(const float *__restrict__ model, int L,
                                                       const float *__restrict__ rotm, const int32_t *__restrict__ syn_rot,
-                                                      const float *__restrict__ axyz, int Pa, int Pp,
-                                                      const double *__restrict__ sumW, float photons, uint64_t seed,
-                                                      float *__restrict__ counts)
+                                                      const float *__restrict__ axyz, int Pa, int Pp, float photons,
+                                                      uint64_t seed, float *__restrict__ counts)
 {
+    __shared__ double red[4];
     const int b = blockIdx.x, s = syn_rot[b];
     float m[9];
     load_rot(rotm, s, m);
-    const double sw = sumW[s];
-    const float scale = sw > 0.0 ? (float)(photons / sw) : 0.0f;
+    float *row = counts + (size_t)b * Pp;
+    double sw = 0.0;
     for (int j = threadIdx.x; j < Pp; j += 256) {
+        float W = 0.0f;
+        if (j < Pa) W = fmaxf(sample<INTERP>(model, L, m, axyz[j], axyz[Pp + j], axyz[2 * Pp + j]), 0.0f);
+        row[j] = W;
+        sw += (double)W;
+    }
+    for (int o = 32; o > 0; o >>= 1) sw += __shfl_xor(sw, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sw;
+    __syncthreads();
+    const double tot = red[0] + red[1] + red[2] + red[3];
+    const float scale = tot > 0.0 ? (float)(photons / tot) : 0.0f;
+    for (int j = threadIdx.x; j < Pp; j += 256) {
+        const float W = row[j];
         float k = 0.0f;
-        if (j < Pa) {
-            const float W = sample<INTERP>(model, L, m, axyz[j], axyz[Pp + j], axyz[2 * Pp + j]);
-            if (W > 0.0f) {
-                DevRng r{seed ^ ((uint64_t)b * 0xD1B54A32D192ED03ull) ^ ((uint64_t)j * 0xABC98388FB8FAC03ull)};
-                k = poisson_dev(W * scale, r);
-            }
+        if (W > 0.0f) {
+            DevRng r{seed ^ ((uint64_t)b * 0xD1B54A32D192ED03ull) ^ ((uint64_t)j * 0xABC98388FB8FAC03ull)};
+            k = poisson_dev(W * scale, r);
         }
-        counts[(size_t)b * Pp + j] = k;
+        row[j] = k;
     }
 }
 
@@ -530,10 +540,10 @@ hipError_t mlp_synth_counts(const emc_ctx *c, int n, float photons, uint64_t see
     const Proposal *pr = c->prop;
     if (c->opt.interp == EMC_INTERP_NEAREST)
         hipLaunchKernelGGL(k_synth_counts<EMC_INTERP_NEAREST>, dim3(n), dim3(256), 0, st, c->model, c->L, c->rotm,
-                           pr->syn_rot, c->axyz, c->Pa, c->Pp, c->sumW, photons, seed, pr->syn_counts);
+                           pr->syn_rot, c->axyz, c->Pa, c->Pp, photons, seed, pr->syn_counts);
     else
         hipLaunchKernelGGL(k_synth_counts<EMC_INTERP_TRILINEAR>, dim3(n), dim3(256), 0, st, c->model, c->L, c->rotm,
-                           pr->syn_rot, c->axyz, c->Pa, c->Pp, c->sumW, photons, seed, pr->syn_counts);
+                           pr->syn_rot, c->axyz, c->Pa, c->Pp, photons, seed, pr->syn_counts);
     return hipGetLastError();
 }
 

@@ -380,13 +380,18 @@ emc_status proposal_targets(emc_ctx *c, bool measure, double *captured)
     return EMC_OK;
 }
 
-emc_status proposal_train(emc_ctx *c, int epochs, double *loss)
+emc_status proposal_train(emc_ctx *c, int epochs, bool synth_only, double *loss)
 {
     Proposal *p = c->prop;
     hipStream_t st = c->stream;
-    if (!p->targets_valid) return pfail(c, EMC_E_STATE, "proposal_train: no responsibility targets yet");
+    if (!synth_only && !p->targets_valid) return pfail(c, EMC_E_STATE, "proposal_train: no responsibility targets yet");
     const int I = c->I, B = p->o.batch, n_syn = p->n_syn;
-    const int64_t N = (int64_t)I + n_syn;
+    if (synth_only && n_syn == 0) {
+        if (loss) *loss = -1.0;
+        return EMC_OK;
+    }
+    // training items: patterns 0..I-1 (cell targets) then synthetic patterns I..I+n_syn-1
+    const int64_t N = synth_only ? n_syn : (int64_t)I + n_syn, first = synth_only ? I : 0;
     EmcRng rng(p->rng_state);
     std::vector<int32_t> idx(N + B), srot(std::max(n_syn, 1));
     double last = -1.0;
@@ -400,7 +405,7 @@ emc_status proposal_train(emc_ctx *c, int epochs, double *loss)
             P_HIP(c, mlp_features(p->syn_counts, nullptr, nullptr, c->Pp, c->Pa, c->pix, c->L, n_syn, pad, p->raw, st));
             P_HIP(c, mlp_standardise(p->raw, n_syn, pad, p->mu, p->istd, p->Xs, st));
         }
-        for (int64_t k = 0; k < N; ++k) idx[k] = (int32_t)k;
+        for (int64_t k = 0; k < N; ++k) idx[k] = (int32_t)(first + k);
         for (int64_t k = N - 1; k > 0; --k) std::swap(idx[k], idx[rng.next() % (uint64_t)(k + 1)]);
         for (int k = 0; k < B; ++k) idx[N + k] = idx[0];
         P_HIP(c, hipMemcpyAsync(p->bidx, idx.data(), sizeof(int32_t) * (size_t)(N + B), hipMemcpyHostToDevice, st));
@@ -437,6 +442,7 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->epochs = 4;
     o->batch = 256;
     o->synth_patterns = -1;
+    o->refresh_epochs = 1;
     o->min_mass = 0.99f;
     o->lr = 1e-3f;
     o->max_fallback = 0.1f;
@@ -594,6 +600,7 @@ extern "C" emc_status emc_proposal_enable(emc_ctx *c, const float *coarse, int32
     if (o.top_k < 1 || o.top_k > 64 || o.top_k > n_coarse || o.hidden < 64 || o.hidden % 64 || o.hidden > 4096 ||
         o.batch < 64 || o.batch % 64 || o.batch > 65536 || o.epochs < 0 || o.warmup_full < 1 || o.full_every < 1 ||
         !(o.lr > 0.0f) || !(o.min_mass >= 0.0f) || !(o.max_fallback >= 0.0f) || o.synth_patterns < -1 ||
+        o.refresh_epochs < 0 || o.refresh_epochs > 1000 ||
         o.synth_patterns > (1 << 22))
         return pfail(c, EMC_E_INVALID, "proposal_enable: bad option value");
     P_HIP(c, hipSetDevice(c->opt.device));
@@ -642,7 +649,7 @@ extern "C" emc_status emc_proposal_train(emc_ctx *c, int32_t epochs, double *los
     if (!c->has_resp) return pfail(c, EMC_E_STATE, "proposal_train: needs full-search responsibilities");
     P_HIP(c, hipSetDevice(c->opt.device));
     P_TRY(proposal_targets(c, false, nullptr));
-    return proposal_train(c, epochs, loss);
+    return proposal_train(c, epochs, false, loss);
 }
 
 extern "C" emc_status emc_proposal_features(emc_ctx *c, float *raw)

@@ -236,8 +236,10 @@ emc_status emc_synth_uniform_model(int32_t L, uint64_t seed, float *model);
  * each pattern, only the fine rotations of its top_k cells (all cells when the predicted mass in
  * them is below min_mass) as a grouped MFMA contraction; its softmax, M-step sum and insertion
  * see P = 0 elsewhere.  Iterations 0 .. warmup_full-1 and every full_every-th iteration after
- * them search fully, train the network and measure the captured mass.  Needs an MFMA precision,
- * TRUE_POISSON, slice_chunk covering all rotations and counts <= 256. */
+ * them search fully, train the network and measure the captured mass; after a proposal
+ * iteration's compress the network takes refresh_epochs passes over patterns sliced from the new
+ * model.  Needs an MFMA precision, TRUE_POISSON, slice_chunk covering all rotations and counts
+ * <= 256. */
 typedef struct emc_proposal_options {
     int32_t top_k;          /* cells kept per pattern (8) */
     int32_t warmup_full;    /* leading full-search iterations (3) */
@@ -246,7 +248,8 @@ typedef struct emc_proposal_options {
     int32_t epochs;         /* passes over the training set after each full search (4) */
     int32_t batch;          /* minibatch rows, multiple of 64 (256) */
     int32_t synth_patterns; /* model-sliced training patterns per pass; -1 = n_patterns / 4 */
-    int32_t reserved0;
+    int32_t refresh_epochs; /* passes over fresh model-sliced patterns after each proposal
+                               iteration's compress, so the network follows the model (1) */
     float min_mass;         /* per-pattern predicted mass below which it is searched fully (0.99) */
     float lr;               /* Adam step size (1e-3; beta 0.9 / 0.999, eps 1e-8) */
     float max_fallback;     /* fraction of fully searched patterns above which the iteration
