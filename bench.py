@@ -5,8 +5,11 @@ Workload (default): 64^3 model, 25,680 rotations (n=8), 20,000 synthetic Poisson
 one full EMC iteration (slice -> logL -> responsibilities -> maximize -> compress) with every
 input already resident in HBM.  `value` is the split-BF16 MFMA path; the plain FP32 path is timed
 beside it (`fp32`), and the FP64 CPU oracle on a rotation subset is the `cpu_baseline`.
+`proposal` compares the learned orientation proposal with the full search at 204,960 rotations
+and 50,000 patterns (64^3 model, 2,000 photons): the same iterations from the same start.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload NAME] [--no-cpu] [--no-fp32]
+                    [--no-proposal] [--proposal-iters 13] [--accuracy] [--variants]
 """
 from __future__ import annotations
 
@@ -143,7 +146,7 @@ def accuracy(d, n_sub_logl: int, n_model: int):
     return res
 
 
-def run_proposal(name: str, iters: int, steps_note: str = ""):
+def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: bool = False):
     """Learned proposal at 204,960 fine rotations (n=16) / 420 coarse cells (n=2), 50,000
     patterns: the same iterations from the same start with the full search only (reference) and
     with the proposal policy; per-iteration times, captured mass, model relative L2 vs full."""
@@ -174,9 +177,10 @@ def run_proposal(name: str, iters: int, steps_note: str = ""):
             log(f"[proposal] full it {it}: {full_ms[-1]:.1f} ms (logL GEMM {s['stage_ms']['logl_gemm']:.1f}, "
                 f"M-step GEMM {s['stage_ms']['mstep_gemm']:.1f})")
     rows = []
+    prev_top = None
     t = time.perf_counter()
     with engine() as e:
-        e.enable_proposal(cq)
+        e.enable_proposal(cq, **(options or {}))
         log(f"[proposal] proposal engine ready in {time.perf_counter() - t:.1f}s")
         for it in range(iters):
             t0 = time.perf_counter()
@@ -189,6 +193,20 @@ def run_proposal(name: str, iters: int, steps_note: str = ""):
                    "captured_mass": s["captured_mass"], "train_loss": s["train_loss"],
                    "model_rel_l2_vs_full": rel,
                    "stage_ms": {k: round(v, 3) for k, v in s["stage_ms"].items()}}
+            if s["captured_mass"] >= 0:
+                cap = e.proposal_captured()
+                row["captured_quantiles"] = {f"q{q:g}": float(np.quantile(cap, q / 100)) for q in (1, 5, 10, 50)}
+                row["patterns_below"] = {f"{t:g}": float((cap < t).mean()) for t in (0.5, 0.9, 0.99)}
+            if diagnose and s["search"] == "full":
+                # did the missed patterns' posteriors move since the previous full search?
+                tg = e.proposal_cell_targets()
+                top8 = np.argsort(-tg, axis=1)[:, :8]
+                if prev_top is not None and s["captured_mass"] >= 0:
+                    moved = np.take_along_axis(tg, prev_top, 1).sum(1) < 0.5
+                    missed = cap < 0.5
+                    row["diagnosis"] = {"moved": float(moved.mean()), "missed": float(missed.mean()),
+                                        "missed_and_moved": float((missed & moved).mean())}
+                prev_top = top8
             rows.append(row)
             log(f"[proposal] it {it}: {json.dumps(row)}")
     prop_ms = [r["ms"] for r in rows if r["search"] == "proposal"]
@@ -222,10 +240,12 @@ def main():
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
     ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
     ap.add_argument("--accuracy", action="store_true", help="precision against the FP64 oracle (slow)")
-    ap.add_argument("--proposal", action="store_true",
-                    help="also run the learned proposal at 204,960 rotations against the full search")
+    ap.add_argument("--no-proposal", action="store_true",
+                    help="skip the learned proposal at 204,960 rotations against the full search")
     ap.add_argument("--proposal-only", action="store_true", help="only the learned-proposal comparison")
-    ap.add_argument("--proposal-iters", type=int, default=8)
+    ap.add_argument("--proposal-iters", type=int, default=13)
+    ap.add_argument("--proposal-options", default="{}", help="JSON object of emc_proposal_options overrides")
+    ap.add_argument("--diagnose", action="store_true", help="proposal: where the missed responsibility mass went")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -238,7 +258,8 @@ def main():
     info = emc_amd.device_info(0)
     log(f"[bench] device {info['name']}, {info['n_cu']} CUs, {info['hbm_bytes'] / 2**30:.0f} GiB")
     if args.proposal_only:
-        print(json.dumps({"proposal": run_proposal("L64_S204960_I50000", args.proposal_iters)}))
+        print(json.dumps({"proposal": run_proposal("L64_S204960_I50000", args.proposal_iters,
+                                                   json.loads(args.proposal_options), args.diagnose)}))
         return
     t = time.perf_counter()
     d = wl.make(args.workload)
@@ -276,7 +297,13 @@ def main():
     acc = accuracy(d, 400, 4000) if args.accuracy else None
     if acc:
         log(f"[bench] accuracy {json.dumps(acc)}")
-    prop = run_proposal("L64_S204960_I50000", args.proposal_iters) if args.proposal else None
+    prop = None
+    if not args.no_proposal:
+        try:
+            prop = run_proposal("L64_S204960_I50000", args.proposal_iters, json.loads(args.proposal_options))
+        except Exception as ex:  # reported, not fatal: the headline line above does not depend on it
+            log(f"[bench] proposal comparison failed: {ex}")
+            prop = {"error": str(ex)}
 
     cpu = None
     if not args.no_cpu:

@@ -270,6 +270,12 @@ class Engine:
         self._c(lib().emc_proposal_predict(self._h, cells.ctypes.data, mass.ctypes.data))
         return cells, mass
 
+    def proposal_captured(self):
+        """Per-pattern captured mass of the last full-search iteration that measured it."""
+        cap = np.zeros(self.I, np.float64)
+        self._c(lib().emc_proposal_captured(self._h, cap))
+        return cap
+
     def proposal_cell_targets(self):
         t = np.zeros((self.I, self.n_cells), np.float32)
         self._c(lib().emc_proposal_cell_targets(self._h, t))

@@ -48,7 +48,7 @@ class IterStats(C.Structure):
 class ProposalOptions(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("top_k", "warmup_full", "full_every", "hidden", "epochs", "batch",
                                           "synth_patterns", "refresh_epochs")] + \
-               [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "reserved1")] + \
+               [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "refresh_fraction")] + \
                [("seed", C.c_uint64)]
 
 
@@ -129,6 +129,7 @@ SIGNATURES = {
     "emc_proposal_disable": (C.c_int, [ctxp]),
     "emc_proposal_predict": (C.c_int, [ctxp, vp, vp]),
     "emc_proposal_cell_targets": (C.c_int, [ctxp, f32p]),
+    "emc_proposal_captured": (C.c_int, [ctxp, f64p]),
     "emc_proposal_train": (C.c_int, [ctxp, C.c_int32, C.POINTER(C.c_double)]),
     "emc_proposal_features": (C.c_int, [ctxp, f32p]),
     "emc_mlp_get_params": (C.c_int, [ctxp, vp, vp, vp, vp, vp, vp]),

@@ -79,6 +79,7 @@ struct Proposal {
     std::vector<int32_t> topc_h;
     std::vector<float> pmass_h;
     bool pred_valid = false;
+    std::vector<double> cap_h;     // per-pattern captured mass of the last measurement
     // grouped work lists (built on the host from the prediction)
     int64_t n_slots = 0, n_gslots = 0, n_blk = 0;  // slots, padded slots, block elements
     int n_fallback = 0;

@@ -31,13 +31,11 @@ struct BinGeom {
     int L, nb, S, Pa, Pp, chunk, interp;
 };
 
-__device__ __forceinline__ bool item_cell(const float *__restrict__ rotm, const float *__restrict__ axyz,
-                                          const BinGeom &g, int s, int j, int &bx, int &by, int &bz, float &fx,
-                                          float &fy, float &fz)
+// Voxel cell of active pixel j under rotation matrix m (nearest: the voxel; trilinear: the base
+// corner and fractions); false when out of bounds.
+__device__ __forceinline__ bool item_cell_m(const float *m, const float *__restrict__ axyz, const BinGeom &g, int j,
+                                            int &bx, int &by, int &bz, float &fx, float &fy, float &fz)
 {
-    float m[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) m[k] = rotm[12 * (size_t)s + k];
     float ux, uy, uz;
     rot_point(m, axyz[j], axyz[g.Pp + j], axyz[2 * g.Pp + j], model_c(g.L), ux, uy, uz);
     if (g.interp == EMC_INTERP_NEAREST) {
@@ -48,6 +46,15 @@ __device__ __forceinline__ bool item_cell(const float *__restrict__ rotm, const 
         return bx >= 0 && bx < g.L && by >= 0 && by < g.L && bz >= 0 && bz < g.L;  // emc_atomic.cu:67-69
     }
     return tri_axis(ux, g.L, bx, fx) && tri_axis(uy, g.L, by, fy) && tri_axis(uz, g.L, bz, fz);
+}
+
+__device__ __forceinline__ bool item_cell(const float *__restrict__ rotm, const float *__restrict__ axyz,
+                                          const BinGeom &g, int s, int j, int &bx, int &by, int &bz, float &fx,
+                                          float &fy, float &fz)
+{
+    float m[9];
+    load_rot(rotm, s, m);
+    return item_cell_m(m, axyz, g, j, bx, by, bz, fx, fy, fz);
 }
 
 __global__ void k_bin_keys(const float *__restrict__ rotm, const float *__restrict__ axyz, BinGeom g,
@@ -95,14 +102,27 @@ __global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict
     for (int t = begin + threadIdx.x; t < end; t += 256) {
         const uint32_t idx = vals[t];
         const int s = (int)(idx / (uint32_t)g.Pa), j = (int)(idx % (uint32_t)g.Pa);
-        const double Td = T[s];
+        // Items are ordered by (s, j), so a wavefront usually holds one rotation: then its matrix
+        // and slice weight are read once through the scalar unit instead of per lane.
+        const int sw = __builtin_amdgcn_readfirstlane(s);
+        float m[9];
+        double Td;
+        if (__ballot(s != sw) == 0) {
+            const float *rm = rotm + 12 * (size_t)sw;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) m[k] = rm[k];
+            Td = T[sw];
+        } else {
+            load_rot(rotm, s, m);
+            Td = T[s];
+        }
         if (!(Td > 1e-10)) continue;  // emc_atomic.cu:190
         const float val = U[(size_t)(s - s0) * g.Pp + j];
         if (!(val >= 0.0f)) continue;  // emc_atomic.cu:56
         const float Ts = (float)Td;
         int bx, by, bz;
         float fx, fy, fz;
-        item_cell(rotm, axyz, g, s, j, bx, by, bz, fx, fy, fz);
+        item_cell_m(m, axyz, g, j, bx, by, bz, fx, fy, fz);
         const int lx = bx - ox, ly = by - oy, lz = bz - oz;
         if (INTERP == EMC_INTERP_NEAREST) {
             const int l = (lz * kHalo + ly) * kHalo + lx;

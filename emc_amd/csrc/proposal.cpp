@@ -376,6 +376,7 @@ emc_status proposal_targets(emc_ctx *c, bool measure, double *captured)
         double s = 0.0;
         for (double v : cap) s += v;
         if (captured) *captured = s / c->I;
+        p->cap_h = std::move(cap);
     }
     return EMC_OK;
 }
@@ -385,7 +386,10 @@ emc_status proposal_train(emc_ctx *c, int epochs, bool synth_only, double *loss)
     Proposal *p = c->prop;
     hipStream_t st = c->stream;
     if (!synth_only && !p->targets_valid) return pfail(c, EMC_E_STATE, "proposal_train: no responsibility targets yet");
-    const int I = c->I, B = p->o.batch, n_syn = p->n_syn;
+    const int I = c->I, B = p->o.batch;
+    // a refresh pass draws refresh_fraction of the synthetic patterns
+    const int n_syn = synth_only ? std::max(1, (int)std::lround(p->n_syn * (double)p->o.refresh_fraction)) * (p->n_syn > 0)
+                                 : p->n_syn;
     if (synth_only && n_syn == 0) {
         if (loss) *loss = -1.0;
         return EMC_OK;
@@ -443,6 +447,7 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->batch = 256;
     o->synth_patterns = -1;
     o->refresh_epochs = 1;
+    o->refresh_fraction = 0.25f;
     o->min_mass = 0.99f;
     o->lr = 1e-3f;
     o->max_fallback = 0.1f;
@@ -600,7 +605,7 @@ extern "C" emc_status emc_proposal_enable(emc_ctx *c, const float *coarse, int32
     if (o.top_k < 1 || o.top_k > 64 || o.top_k > n_coarse || o.hidden < 64 || o.hidden % 64 || o.hidden > 4096 ||
         o.batch < 64 || o.batch % 64 || o.batch > 65536 || o.epochs < 0 || o.warmup_full < 1 || o.full_every < 1 ||
         !(o.lr > 0.0f) || !(o.min_mass >= 0.0f) || !(o.max_fallback >= 0.0f) || o.synth_patterns < -1 ||
-        o.refresh_epochs < 0 || o.refresh_epochs > 1000 ||
+        o.refresh_epochs < 0 || o.refresh_epochs > 1000 || !(o.refresh_fraction > 0.0f && o.refresh_fraction <= 1.0f) ||
         o.synth_patterns > (1 << 22))
         return pfail(c, EMC_E_INVALID, "proposal_enable: bad option value");
     P_HIP(c, hipSetDevice(c->opt.device));
@@ -623,6 +628,16 @@ extern "C" emc_status emc_proposal_predict(emc_ctx *c, int32_t *cells, float *ma
     const Proposal *p = c->prop;
     if (cells) std::memcpy(cells, p->topc_h.data(), sizeof(int32_t) * p->topc_h.size());
     if (mass) std::memcpy(mass, p->pmass_h.data(), sizeof(float) * c->I);
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_proposal_captured(emc_ctx *c, double *cap)
+{
+    P_TRY(need_prop(c, "proposal_captured"));
+    if (!cap) return pfail(c, EMC_E_INVALID, "proposal_captured: null argument");
+    const Proposal *p = c->prop;
+    if ((int)p->cap_h.size() != c->I) return pfail(c, EMC_E_STATE, "proposal_captured: no measurement yet");
+    std::memcpy(cap, p->cap_h.data(), sizeof(double) * c->I);
     return EMC_OK;
 }
 

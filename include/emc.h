@@ -254,7 +254,7 @@ typedef struct emc_proposal_options {
     float lr;               /* Adam step size (1e-3; beta 0.9 / 0.999, eps 1e-8) */
     float max_fallback;     /* fraction of fully searched patterns above which the iteration
                                runs a full search instead (0.1) */
-    float reserved1;
+    float refresh_fraction; /* share of synth_patterns drawn for each refresh pass (0.25) */
     uint64_t seed;          /* initial weights, shuffling, synthetic patterns */
 } emc_proposal_options;
 
@@ -271,6 +271,9 @@ emc_status emc_proposal_disable(emc_ctx *ctx);
 /* Network prediction for every pattern: cells [I][top_k] (ascending), mass [I].  Either may be
  * NULL. */
 emc_status emc_proposal_predict(emc_ctx *ctx, int32_t *cells, float *mass);
+/* Per-pattern captured mass of the last full-search iteration that measured it (the
+ * emc_iter_stats.captured_mass average): cap [I]. */
+emc_status emc_proposal_captured(emc_ctx *ctx, double *cap);
 /* Cell sums of the current full responsibilities: targets [I][n_coarse]. */
 emc_status emc_proposal_cell_targets(emc_ctx *ctx, float *targets);
 /* Train on the current full responsibilities (cell targets) plus synthetic patterns for
