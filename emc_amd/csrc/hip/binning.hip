@@ -25,7 +25,7 @@ namespace {
 
 constexpr int kBrick = 8;
 constexpr int kHalo = kBrick + 1;
-constexpr int kUnit = 4096;  // items per workgroup
+constexpr int kUnit = 8192;  // items per workgroup
 
 struct BinGeom {
     int L, nb, S, Pa, Pp, chunk, interp;
@@ -83,6 +83,24 @@ __global__ void k_bin_count(const uint32_t *__restrict__ keys, int64_t n, uint32
     }
 }
 
+// Second sort key of the items of unit u: u * 512 + the base voxel's index inside its 8^3 brick.
+__global__ __launch_bounds__(256) void k_unit_keys(const int32_t *__restrict__ units, const uint32_t *__restrict__ vals,
+                                                   const float *__restrict__ rotm, const float *__restrict__ axyz,
+                                                   BinGeom g, uint32_t *__restrict__ keys)
+{
+    const int u = blockIdx.x;
+    const int begin = units[3 * u + 1], end = units[3 * u + 2];
+    for (int t = begin + threadIdx.x; t < end; t += 256) {
+        const uint32_t idx = vals[t];
+        const int s = (int)(idx / (uint32_t)g.Pa), j = (int)(idx % (uint32_t)g.Pa);
+        int bx = 0, by = 0, bz = 0;
+        float fx, fy, fz;
+        item_cell(rotm, axyz, g, s, j, bx, by, bz, fx, fy, fz);  // valid: only valid pairs are listed
+        const int lv = ((bz % kBrick) * kBrick + by % kBrick) * kBrick + bx % kBrick;
+        keys[t] = (uint32_t)u * (kBrick * kBrick * kBrick) + (uint32_t)lv;
+    }
+}
+
 template <int INTERP>
 __global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict__ units, const uint32_t *__restrict__ vals,
                                                        const float *__restrict__ U, int s0, const double *__restrict__ T,
@@ -99,50 +117,58 @@ __global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict
         accW[k] = 0.0f;
     }
     __syncthreads();
-    for (int t = begin + threadIdx.x; t < end; t += 256) {
+    // A unit's items are sorted by base voxel, so each thread walks a contiguous run of them and
+    // sums the corner contributions in registers while the base voxel repeats; only at a change
+    // of voxel (and at the end) do the sums go to the LDS brick with atomics.
+    constexpr int NC = INTERP == EMC_INTERP_NEAREST ? 1 : 8;
+    const int per = (end - begin + 255) / 256;
+    const int t0 = begin + threadIdx.x * per, t1 = min(end, t0 + per);
+    float sm[NC], sw[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) sm[k] = sw[k] = 0.0f;
+    int cur = -1;
+    auto flush = [&]() {
+        if (cur < 0) return;
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            const int l = cur + ((k >> 2) * kHalo + ((k >> 1) & 1)) * kHalo + (k & 1);
+            atomicAdd(accM + l, sm[k]);
+            atomicAdd(accW + l, sw[k]);
+            sm[k] = sw[k] = 0.0f;
+        }
+    };
+    for (int t = t0; t < t1; ++t) {
         const uint32_t idx = vals[t];
         const int s = (int)(idx / (uint32_t)g.Pa), j = (int)(idx % (uint32_t)g.Pa);
-        // Items are ordered by (s, j), so a wavefront usually holds one rotation: then its matrix
-        // and slice weight are read once through the scalar unit instead of per lane.
-        const int sw = __builtin_amdgcn_readfirstlane(s);
-        float m[9];
-        double Td;
-        if (__ballot(s != sw) == 0) {
-            const float *rm = rotm + 12 * (size_t)sw;
-#pragma unroll
-            for (int k = 0; k < 9; ++k) m[k] = rm[k];
-            Td = T[sw];
-        } else {
-            load_rot(rotm, s, m);
-            Td = T[s];
-        }
+        const double Td = T[s];
         if (!(Td > 1e-10)) continue;  // emc_atomic.cu:190
         const float val = U[(size_t)(s - s0) * g.Pp + j];
         if (!(val >= 0.0f)) continue;  // emc_atomic.cu:56
         const float Ts = (float)Td;
+        float m[9];
+        load_rot(rotm, s, m);
         int bx, by, bz;
         float fx, fy, fz;
         item_cell_m(m, axyz, g, j, bx, by, bz, fx, fy, fz);
-        const int lx = bx - ox, ly = by - oy, lz = bz - oz;
+        const int l = ((bz - oz) * kHalo + (by - oy)) * kHalo + (bx - ox);
+        if (l != cur) {
+            flush();
+            cur = l;
+        }
         if (INTERP == EMC_INTERP_NEAREST) {
-            const int l = (lz * kHalo + ly) * kHalo + lx;
-            atomicAdd(accM + l, val);
-            atomicAdd(accW + l, Ts);
+            sm[0] += val;
+            sw[0] += Ts;
         } else {
             const float gw[2][3] = {{1.0f - fx, 1.0f - fy, 1.0f - fz}, {fx, fy, fz}};
 #pragma unroll
-            for (int dz = 0; dz < 2; ++dz)
-#pragma unroll
-                for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-                    for (int dx = 0; dx < 2; ++dx) {
-                        const float w = gw[dx][0] * gw[dy][1] * gw[dz][2];
-                        const int l = ((lz + dz) * kHalo + ly + dy) * kHalo + lx + dx;
-                        atomicAdd(accM + l, w * val);
-                        atomicAdd(accW + l, w * Ts);
-                    }
+            for (int k = 0; k < NC; ++k) {
+                const float w = gw[k & 1][0] * gw[(k >> 1) & 1][1] * gw[k >> 2][2];
+                sm[k] += w * val;
+                sw[k] += w * Ts;
+            }
         }
     }
+    flush();
     __syncthreads();
     for (int k = threadIdx.x; k < kHalo * kHalo * kHalo; k += 256) {
         const float a = accM[k], b = accW[k];
@@ -239,6 +265,34 @@ hipError_t build_bins(emc_ctx *c, std::string &err)
         chk(hipMalloc((void **)&c->bin_units, sizeof(int32_t) * units.size()));
         if (e == hipSuccess)
             chk(hipMemcpy(c->bin_units, units.data(), sizeof(int32_t) * units.size(), hipMemcpyHostToDevice));
+        // within each unit, order the items by base voxel (stable: then by (s, j)), so the
+        // insertion can sum runs of one voxel in registers
+        const uint32_t n_units = (uint32_t)(units.size() / 3);
+        const uint64_t nkeys2 = (uint64_t)n_units * kBrick * kBrick * kBrick;
+        int end_bit2 = 1;
+        while (end_bit2 < 32 && ((uint64_t)1 << end_bit2) <= nkeys2) ++end_bit2;
+        if (nkeys2 >= ((uint64_t)1 << 32)) chk(hipErrorInvalidValue);
+        // items outside every unit (no valid voxel) keep the largest key and stay at the end
+        const uint32_t sentinel = (uint32_t)(((uint64_t)1 << end_bit2) - 1);
+        if (e == hipSuccess) chk(hipMemsetD32Async((hipDeviceptr_t)kin, (int)sentinel, (size_t)n, st));
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_unit_keys, dim3(n_units), dim3(256), 0, st, c->bin_units, c->bin_vals, c->rotm, c->axyz,
+                               g, kin);
+            chk(hipGetLastError());
+        }
+        if (e == hipSuccess) chk(hipMemcpyAsync(vin, c->bin_vals, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        size_t tmp2 = 0;
+        if (e == hipSuccess)
+            chk(rocprim::radix_sort_pairs(nullptr, tmp2, kin, kout, vin, c->bin_vals, (size_t)n, 0, end_bit2, st));
+        if (e == hipSuccess && tmp2 > tmp_bytes) {
+            (void)hipFree(tmp);
+            tmp = nullptr;
+            chk(hipMalloc(&tmp, tmp2));
+            tmp_bytes = tmp2;
+        }
+        if (e == hipSuccess)
+            chk(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, c->bin_vals, (size_t)n, 0, end_bit2, st));
+        if (e == hipSuccess) chk(hipStreamSynchronize(st));
     }
     (void)hipFree(kin);
     (void)hipFree(vin);
