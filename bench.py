@@ -183,6 +183,9 @@ def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: b
         e.enable_proposal(cq, **(options or {}))
         log(f"[proposal] proposal engine ready in {time.perf_counter() - t:.1f}s")
         for it in range(iters):
+            if diagnose:  # the cells the iteration will search (outside the timed region)
+                rec_before = e.proposal_recall()
+                pred_before, pm_before = e.proposal_predict()
             t0 = time.perf_counter()
             s = e.iterate(it)
             ms = (time.perf_counter() - t0) * 1e3
@@ -201,12 +204,58 @@ def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: b
                 # did the missed patterns' posteriors move since the previous full search?
                 tg = e.proposal_cell_targets()
                 top8 = np.argsort(-tg, axis=1)[:, :8]
+                if s["captured_mass"] >= 0:
+                    # capture of the predicted + recalled cells alone, and with the patterns whose
+                    # predicted mass is below t searched fully, for a range of thresholds t
+                    sel = np.zeros_like(tg, dtype=bool)
+                    np.put_along_axis(sel, pred_before.astype(np.int64), True, 1)
+                    if rec_before.shape[1]:
+                        rr = rec_before.astype(np.int64)
+                        ok = rr >= 0
+                        np.put_along_axis(sel, np.where(ok, rr, pred_before[:, :1]), True, 1)
+                    union = (tg * sel).sum(1)
+                    row["capture_by_threshold"] = {
+                        f"{t:g}": [float((pm_before < t).mean()), float(np.where(pm_before < t, 1.0, union).mean())]
+                        for t in (0.0, 0.5, 0.9, 0.95, 0.98, 0.99, 0.995)}
                 if prev_top is not None and s["captured_mass"] >= 0:
                     moved = np.take_along_axis(tg, prev_top, 1).sum(1) < 0.5
                     missed = cap < 0.5
-                    row["diagnosis"] = {"moved": float(moved.mean()), "missed": float(missed.mean()),
-                                        "missed_and_moved": float((missed & moved).mean())}
-                prev_top = top8
+                    diag = {"moved": float(moved.mean()), "missed": float(missed.mean()),
+                            "missed_and_moved": float((missed & moved).mean())}
+                    if missed.any():
+                        # where the missed patterns' mass is now: their best cell, the recalled
+                        # cells, the network's cells, and how the previous full search saw them
+                        m_idx = np.nonzero(missed)[0]
+                        best = tg[m_idx].argmax(1)
+                        rec = rec_before[m_idx] if rec_before.shape[1] else np.full((m_idx.size, 1), -1)
+                        rec_mass = np.array([tg[i, r[r >= 0]].sum() for i, r in zip(m_idx, rec)])
+                        diag.update({
+                            "missed_best_cell_mass_median": float(np.median(tg[m_idx, best])),
+                            "missed_best_in_recall": float((rec == best[:, None]).any(1).mean()),
+                            "missed_best_in_prediction": float((pred_before[m_idx] == best[:, None]).any(1).mean()),
+                            "missed_recall_mass_mean": float(rec_mass.mean()),
+                            "missed_recalled_cells_mean": float((rec >= 0).sum(1).mean()),
+                            "missed_predicted_mass_median": float(np.median(pm_before[m_idx])),
+                            "missed_prev_mass_in_best_median": float(np.median(prev_tg[m_idx, best])),
+                        })
+                        # is the new best cell next to the previous one, or next to its partner
+                        # under a half turn about the beam axis (q -> q (0,0,0,1)), which gives
+                        # the same flat-detector slice of a centrosymmetric intensity?
+                        pb = prev_tg[m_idx].argmax(1)
+                        qc = cq[pb].astype(np.float64)
+                        partner = np.stack([-qc[:, 3], qc[:, 2], -qc[:, 1], qc[:, 0]], 1)
+                        near_p = np.argsort(-np.abs(partner @ cq.T.astype(np.float64)), 1)[:, :3]
+                        near_s = np.argsort(-np.abs(qc @ cq.T.astype(np.float64)), 1)[:, :4]
+                        diag["missed_best_near_partner"] = float((near_p == best[:, None]).any(1).mean())
+                        diag["missed_best_near_previous"] = float((near_s == best[:, None]).any(1).mean())
+                        # rank of the new best cell in the previous posterior (by mass, 0 = best)
+                        pv = prev_tg[m_idx]
+                        rank = (pv > pv[np.arange(m_idx.size), best][:, None]).sum(1)
+                        for r in (2, 4, 8):
+                            diag[f"missed_best_prev_rank_lt{r}"] = float((rank < r).mean())
+                        diag["missed_best_prev_zero"] = float((pv[np.arange(m_idx.size), best] == 0).mean())
+                    row["diagnosis"] = diag
+                prev_top, prev_tg = top8, tg
             rows.append(row)
             log(f"[proposal] it {it}: {json.dumps(row)}")
     prop_ms = [r["ms"] for r in rows if r["search"] == "proposal"]

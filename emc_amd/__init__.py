@@ -122,7 +122,7 @@ class Engine:
 
     def __init__(self, L: int, S: int, I: int, slice_chunk: int = 0, **options):
         self.L, self.S, self.I = L, S, I
-        self.n_cells = self.top_k = self.hidden = 0  # set by enable_proposal
+        self.n_cells = self.top_k = self.hidden = self.recall = 0  # set by enable_proposal
         self.opt = default_options(**options)
         d = Dims(L, S, I, slice_chunk)
         h = C.c_void_p()
@@ -259,6 +259,7 @@ class Engine:
         self.n_cells = q.shape[0]
         self.top_k = o.top_k
         self.hidden = o.hidden
+        self.recall = o.recall
         self._c(lib().emc_proposal_enable(self._h, q.reshape(-1), q.shape[0], C.byref(o)))
 
     def disable_proposal(self):
@@ -275,6 +276,12 @@ class Engine:
         cap = np.zeros(self.I, np.float64)
         self._c(lib().emc_proposal_captured(self._h, cap))
         return cap
+
+    def proposal_recall(self):
+        """Recalled cells [I][recall] (most massive first, -1 = none)."""
+        cells = np.full((self.I, self.recall), -1, np.int32)
+        self._c(lib().emc_proposal_recall(self._h, cells.ctypes.data))
+        return cells
 
     def proposal_cell_targets(self):
         t = np.zeros((self.I, self.n_cells), np.float32)

@@ -47,7 +47,7 @@ class IterStats(C.Structure):
 
 class ProposalOptions(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("top_k", "warmup_full", "full_every", "hidden", "epochs", "batch",
-                                          "synth_patterns", "refresh_epochs")] + \
+                                          "synth_patterns", "refresh_epochs", "recall", "train_patterns")] + \
                [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "refresh_fraction")] + \
                [("seed", C.c_uint64)]
 
@@ -128,6 +128,7 @@ SIGNATURES = {
     "emc_proposal_enable": (C.c_int, [ctxp, f32p, C.c_int32, C.POINTER(ProposalOptions)]),
     "emc_proposal_disable": (C.c_int, [ctxp]),
     "emc_proposal_predict": (C.c_int, [ctxp, vp, vp]),
+    "emc_proposal_recall": (C.c_int, [ctxp, vp]),
     "emc_proposal_cell_targets": (C.c_int, [ctxp, f32p]),
     "emc_proposal_captured": (C.c_int, [ctxp, f64p]),
     "emc_proposal_train": (C.c_int, [ctxp, C.c_int32, C.POINTER(C.c_double)]),

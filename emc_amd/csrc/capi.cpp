@@ -202,7 +202,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
-    dfree(c->bin_vals); dfree(c->bin_units);
+    dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -801,13 +801,10 @@ extern "C" emc_status emc_normalize_model(emc_ctx *c, double *model_avg)
 
 // ---- proposal iteration stages (the dense stages restricted to each pattern's selected cells) --
 
-static emc_status sparse_logl_enqueue(emc_ctx *c, float sigma)
+// The grouped contraction; the slices were enqueued before the work lists were built.
+static emc_status sparse_logl_enqueue(emc_ctx *c)
 {
     hipStream_t st = c->stream;
-    EMC_TRY(slice_prepare(c, sigma));
-    ev_start(c, EMC_STAGE_LOGL);
-    HIP_TRY(c, launch_slice_bf16(c, 0, c->S, st));
-    gev_rec(c, 0, 0, 0);
     HIP_TRY(c, launch_grouped_logl(c, st));
     gev_rec(c, 0, 0, 1);
     ev_stop(c, EMC_STAGE_LOGL);
@@ -828,7 +825,6 @@ static emc_status sparse_mstep_enqueue(emc_ctx *c)
     ev_start(c, EMC_STAGE_MSTEP);
     HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
     HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
-    HIP_TRY(c, launch_build_ktg(c, st));
     gev_rec(c, 1, 0, 0);
     HIP_TRY(c, launch_grouped_mstep(c, st));
     gev_rec(c, 1, 0, 1);
@@ -852,7 +848,7 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
     // search policy: full search until the network has trained, during the warm-up, on every
     // full_every-th iteration after it, and when too many patterns would fall back
     Proposal *pr = c->prop;
-    bool full = true, predicted = false;
+    bool full = true, predicted = false, sliced = false;
     double captured = -1.0, train_loss = -1.0, pmass_mean = 0.0;
     int n_fb = 0;
     if (pr) {
@@ -860,14 +856,24 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         full = pr->passes == 0 || t < w || (t - w + 1) % pr->o.full_every == 0;
         if (pr->passes > 0) {
             ev_start(c, EMC_STAGE_PROPOSE);
-            EMC_TRY(proposal_predict(c));
+            EMC_TRY(proposal_predict_enqueue(c));
+            ev_stop(c, EMC_STAGE_PROPOSE);
+            if (!full) {
+                // the slices and log-slice operands do not depend on the prediction: the device
+                // computes them while the host builds the work lists
+                EMC_TRY(slice_prepare(c, (float)sigma));
+                ev_start(c, EMC_STAGE_LOGL);
+                HIP_TRY(c, launch_slice_bf16(c, 0, c->S, c->stream));
+                gev_rec(c, 0, 0, 0);
+                sliced = true;
+            }
+            EMC_TRY(proposal_predict_wait(c));
             predicted = true;
             if (!full) {
                 bool too_many = false;
                 EMC_TRY(proposal_build_lists(c, &too_many));
                 if (too_many) full = true;
             }
-            ev_stop(c, EMC_STAGE_PROPOSE);
             double s = 0.0;
             for (int i = 0; i < c->I; ++i) {
                 s += pr->pmass_h[i];
@@ -877,13 +883,25 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         }
     }
     if (full) {
-        EMC_TRY(logl_enqueue(c, (float)sigma));
+        if (sliced) {  // one chunk covers every rotation while the proposal is enabled
+            HIP_TRY(c, launch_logl_mfma(c, 0, c->S, c->stream));
+            gev_rec(c, 0, 0, 1);
+            ev_stop(c, EMC_STAGE_LOGL);
+        } else {
+            EMC_TRY(logl_enqueue(c, (float)sigma));
+        }
         EMC_TRY(softmax_enqueue(c));
         EMC_TRY(mstep_enqueue(c));
-        if (pr) EMC_TRY(proposal_targets(c, predicted, &captured));  // cell sums of the full P
+        if (pr) {
+            // cell sums of the full P and each cell's largest logit, then the recall from those
+            // (after the capture is measured with the old recall)
+            EMC_TRY(proposal_targets(c, predicted, true, &captured));
+            HIP_TRY(c, launch_recall_dense(c, c->stream));
+        }
     } else {
-        EMC_TRY(sparse_logl_enqueue(c, (float)sigma));
+        EMC_TRY(sparse_logl_enqueue(c));
         EMC_TRY(sparse_softmax_enqueue(c));
+        HIP_TRY(c, launch_recall_grouped(c, c->stream));
         EMC_TRY(sparse_mstep_enqueue(c));
     }
     EMC_TRY(compress_enqueue(c));
@@ -892,8 +910,9 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         // iteration uses: full iterations train on the targets plus those, proposal iterations
         // refresh on those alone
         ev_start(c, EMC_STAGE_TRAIN);
-        if (full) EMC_TRY(proposal_train(c, pr->o.epochs, false, &train_loss));
-        else if (pr->o.refresh_epochs > 0) EMC_TRY(proposal_train(c, pr->o.refresh_epochs, true, &train_loss));
+        if (full) EMC_TRY(proposal_train(c, pr->o.epochs, pr->o.train_patterns != 0, 1.0, &train_loss));
+        else if (pr->o.refresh_epochs > 0)
+            EMC_TRY(proposal_train(c, pr->o.refresh_epochs, false, pr->o.refresh_fraction, &train_loss));
         ev_stop(c, EMC_STAGE_TRAIN);
     }
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL + 1], c->stream);

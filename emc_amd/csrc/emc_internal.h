@@ -63,6 +63,7 @@ struct Proposal {
     double *loss_part = nullptr;
     // training
     float *Tt = nullptr;           // [Cp][Ip] cell sums of the responsibilities
+    float *Mx = nullptr;           // [Cp][Ip] largest logit per cell (when recall > 0)
     bool targets_valid = false;
     uint16_t *Xb = nullptr;        // [batch][D]
     float *tb = nullptr;           // [batch][Cp]
@@ -76,15 +77,19 @@ struct Proposal {
     int32_t *topc = nullptr;       // [Ip][top_k]
     float *pmass = nullptr;        // [Ip]
     double *cap = nullptr;         // [Ip] captured mass per pattern
-    std::vector<int32_t> topc_h;
-    std::vector<float> pmass_h;
+    int32_t *topc_h = nullptr;     // pinned host copies of the last prediction: [I][top_k]
+    float *pmass_h = nullptr;      // [I]
+    hipEvent_t ev_pred = nullptr;  // recorded after the prediction's copies to the host
     bool pred_valid = false;
     std::vector<double> cap_h;     // per-pattern captured mass of the last measurement
+    // recall: cells of largest max logit in each pattern's most recent search, -1 = none
+    int32_t *recall = nullptr;     // [Ip][recall]
+    int32_t *recall_h = nullptr;   // [I][recall] pinned host copy taken with each prediction
     // grouped work lists (built on the host from the prediction)
-    int64_t n_slots = 0, n_gslots = 0, n_blk = 0;  // slots, padded slots, block elements
+    int64_t n_slots = 0, n_blk = 0;  // slots, block elements
     int n_fallback = 0;
     double work_fraction = 0.0;
-    int n_tiles_e = 0, n_tiles_m = 0, n_tiles_sm = 0, n_tiles_kt = 0;
+    int n_tiles_e = 0, n_tiles_m = 0, n_tiles_sm = 0;
     int64_t list_cap = 0;          // int32 elements of `lists`
     int32_t *lists = nullptr;      // device copy of the packed lists below
     int32_t *lists_h = nullptr;    // pinned host staging
@@ -93,9 +98,7 @@ struct Proposal {
     float *raw = nullptr;          // [max(Ip, n_syn)][D] unstandardised features (scratch)
     // offsets (in int32 elements) of each list inside `lists`
     int64_t off_slot_pat = 0, off_sbeg = 0, off_nsp = 0, off_pbeg = 0, off_pslot = 0, off_tile_e = 0,
-            off_tile_m = 0, off_tile_sm = 0, off_tile_kt = 0, off_boff = 0, off_gbeg = 0;
-    int64_t ktg_cap = 0;           // elements of ktg
-    void *ktg = nullptr;           // [sum nsp_c][Pp]... stored per cell as [Pp][nsp_c]
+            off_tile_m = 0, off_tile_sm = 0;
     int64_t slot_cap = 0;          // doubles of slot scratch
     double *slot_ps = nullptr;     // [slots] partial sums
     float *slot_pm = nullptr;      // [slots] partial maxima
@@ -164,7 +167,8 @@ struct emc_ctx {
     double *scal = nullptr;                            // [SC_N] device scalars
     double *scal_h = nullptr;                          // pinned host mirror
     // binned insertion: (s, j) pair indices sorted by (chunk, model brick); work units
-    uint32_t *bin_vals = nullptr;                      // [S * Pa]
+    uint32_t *bin_vals = nullptr;                      // [S * Pa] pair index s * Pa + j, (s, j) order per unit
+    uint32_t *bin_perm = nullptr;                      // [S * Pa] positions in bin_vals, base-voxel order per unit
     int32_t *bin_units = nullptr;                      // [n_units][3] = key, begin, end
     std::vector<int> bin_chunk_units;                  // unit range of each rotation chunk
     bool bins_valid = false;
@@ -222,8 +226,12 @@ hipError_t launch_sum_f64(const double *x, int n, double *out, hipStream_t st);
 // Grouped (per coarse cell) contractions and softmax over the work lists in c->prop.
 hipError_t launch_grouped_logl(const emc_ctx *c, hipStream_t st);
 hipError_t launch_grouped_softmax(const emc_ctx *c, hipStream_t st);
-hipError_t launch_build_ktg(const emc_ctx *c, hipStream_t st);
 hipError_t launch_grouped_mstep(const emc_ctx *c, hipStream_t st);
+// Recalled cells: each pattern's cells of largest max logit among the searched slots (after
+// launch_grouped_softmax) or among all cells (Mx, after a full search).
+hipError_t launch_recall_grouped(const emc_ctx *c, hipStream_t st);
+hipError_t launch_recall_dense(const emc_ctx *c, hipStream_t st);
+constexpr int kMaxRecall = 8;  // largest number of recalled cells per pattern
 
 // Network kernels.  mlp_gemm: C[m,n] = sum_k A(m,k) B(n,k), A/B "col" = stored [k][m] / [k][n];
 // M, N multiples of 64, K of 32.
@@ -244,22 +252,30 @@ hipError_t mlp_adam(float *w, const float *g, float *m, float *v, bf16_t *w16, i
                     hipStream_t st);
 hipError_t mlp_to_bf16(const float *w, int64_t n, bf16_t *w16, hipStream_t st);
 hipError_t mlp_topk(const float *p, int Cp, int C, int rows, int k, int32_t *cells, float *mass, hipStream_t st);
-hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, hipStream_t st);
+// Cell sums Tt of the full responsibilities; with Mx also each cell's largest logit (reads the
+// logits c->R, so only while they are resident: right after a full-search E-step).
+hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, float *Mx, hipStream_t st);
 hipError_t mlp_captured(const emc_ctx *c, double *cap, hipStream_t st);
+// Adds the recalled cells' predicted probability (those not in the top k) to pmass.
+hipError_t mlp_union_mass(const emc_ctx *c, hipStream_t st);
 hipError_t mlp_gather_batch(const emc_ctx *c, const int32_t *idx, int rows, hipStream_t st);
 hipError_t mlp_synth_counts(const emc_ctx *c, int n, float photons, uint64_t seed, hipStream_t st);
 
 // Host side of the proposal (proposal.cpp); each sets c->err on failure.
 void proposal_free(Proposal *p);
-// Network prediction for every pattern into topc / pmass (device) and topc_h / pmass_h.
+// Network prediction for every pattern into topc / pmass (device) and topc_h / pmass_h /
+// recall_h (host): enqueue, then wait for the host copies (the stream may run other work
+// enqueued in between).
+emc_status proposal_predict_enqueue(emc_ctx *c);
+emc_status proposal_predict_wait(emc_ctx *c);
 emc_status proposal_predict(emc_ctx *c);
 // Grouped work lists from the prediction; *too_many = the fallback share exceeds max_fallback.
 emc_status proposal_build_lists(emc_ctx *c, bool *too_many);
 // Cell targets from the dense responsibilities; captured mass of the last prediction if asked.
-emc_status proposal_targets(emc_ctx *c, bool measure, double *captured);
-// Training passes over the targets plus synthetic patterns (or synthetic patterns only, sliced
-// from the current model); loss of the last pass.
-emc_status proposal_train(emc_ctx *c, int epochs, bool synth_only, double *loss);
+emc_status proposal_targets(emc_ctx *c, bool measure, bool logits, double *captured);
+// Training passes over the patterns' cell targets (use_patterns) and syn_fraction of the
+// synthetic patterns, sliced afresh from the current model each pass; loss of the last pass.
+emc_status proposal_train(emc_ctx *c, int epochs, bool use_patterns, double syn_fraction, double *loss);
 
 // BF16 helpers (host and device agree: round to nearest even, finite inputs).
 __host__ __device__ inline bf16_t f2bf(float x)

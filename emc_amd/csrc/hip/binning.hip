@@ -6,10 +6,12 @@
 // model brick (kBrick^3 voxels) that holds their base voxel, per rotation chunk, with a stable
 // radix sort (rocPRIM).  Every iteration then runs one workgroup per slice of a brick's list:
 // it accumulates U[s, j] and T_s with LDS atomics into a (kBrick+1)^3 private copy of the brick
-// (the +1 is the trilinear halo) and flushes it to the model numerator and weight once.  Reads
-// of the item list, of U and of the pixel coordinates are contiguous because the list is
-// ordered by (s, j) within each brick.  This replaces the 16 global atomics per (s, j) of the
-// reference's insert_slices_kernel (emc_atomic.cu:33-85, 183-200) by 16 LDS atomics.
+// (the +1 is the trilinear halo) and flushes it to the model numerator and weight once.  The
+// workgroup first stages its items' U[s, j] and T_s in LDS in (s, j) order, where the reads of U
+// are runs of consecutive pixels; a second, per-unit permutation (by base voxel) then drives the
+// accumulation, so that runs of one voxel are summed in registers before the LDS atomics.  This
+// replaces the 16 global atomics per (s, j) of the reference's insert_slices_kernel
+// (emc_atomic.cu:33-85, 183-200).
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -83,6 +85,12 @@ __global__ void k_bin_count(const uint32_t *__restrict__ keys, int64_t n, uint32
     }
 }
 
+__global__ void k_iota(uint32_t *__restrict__ v, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        v[t] = (uint32_t)t;
+}
+
 // Second sort key of the items of unit u: u * 512 + the base voxel's index inside its 8^3 brick.
 __global__ __launch_bounds__(256) void k_unit_keys(const int32_t *__restrict__ units, const uint32_t *__restrict__ vals,
                                                    const float *__restrict__ rotm, const float *__restrict__ axyz,
@@ -103,11 +111,13 @@ __global__ __launch_bounds__(256) void k_unit_keys(const int32_t *__restrict__ u
 
 template <int INTERP>
 __global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict__ units, const uint32_t *__restrict__ vals,
-                                                       const float *__restrict__ U, int s0, const double *__restrict__ T,
+                                                       const uint32_t *__restrict__ perm, const float *__restrict__ U,
+                                                       int s0, const double *__restrict__ T,
                                                        const float *__restrict__ rotm, const float *__restrict__ axyz,
                                                        BinGeom g, float *__restrict__ Mnum, float *__restrict__ Wt)
 {
     __shared__ float accM[kHalo * kHalo * kHalo], accW[kHalo * kHalo * kHalo];
+    __shared__ float sval[kUnit], sts[kUnit];
     const int u = blockIdx.x;
     const int brick = units[3 * u] % (g.nb * g.nb * g.nb);
     const int begin = units[3 * u + 1], end = units[3 * u + 2];
@@ -116,10 +126,21 @@ __global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict
         accM[k] = 0.0f;
         accW[k] = 0.0f;
     }
+    // Stage U[s, j] and T_s of the unit's items in (s, j) order, where the reads of U are runs
+    // of consecutive pixels; -1 marks an item the insertion skips.
+    for (int k = threadIdx.x; k < end - begin; k += 256) {
+        const uint32_t idx = vals[begin + k];
+        const int s = (int)(idx / (uint32_t)g.Pa), j = (int)(idx % (uint32_t)g.Pa);
+        const double Td = T[s];
+        const float val = U[(size_t)(s - s0) * g.Pp + j];
+        // emc_atomic.cu:190 (T_s > 1e-10) and emc_atomic.cu:56 (U >= 0)
+        sval[k] = (Td > 1e-10 && val >= 0.0f) ? val : -1.0f;
+        sts[k] = (float)Td;
+    }
     __syncthreads();
-    // A unit's items are sorted by base voxel, so each thread walks a contiguous run of them and
-    // sums the corner contributions in registers while the base voxel repeats; only at a change
-    // of voxel (and at the end) do the sums go to the LDS brick with atomics.
+    // perm lists the unit's items by base voxel, so each thread walks a contiguous run of them
+    // and sums the corner contributions in registers while the base voxel repeats; only at a
+    // change of voxel (and at the end) do the sums go to the LDS brick with atomics.
     constexpr int NC = INTERP == EMC_INTERP_NEAREST ? 1 : 8;
     const int per = (end - begin + 255) / 256;
     const int t0 = begin + threadIdx.x * per, t1 = min(end, t0 + per);
@@ -138,13 +159,12 @@ __global__ __launch_bounds__(256) void k_insert_binned(const int32_t *__restrict
         }
     };
     for (int t = t0; t < t1; ++t) {
-        const uint32_t idx = vals[t];
+        const uint32_t pos = perm[t];
+        const float val = sval[pos - begin];
+        if (val < 0.0f) continue;
+        const float Ts = sts[pos - begin];
+        const uint32_t idx = vals[pos];
         const int s = (int)(idx / (uint32_t)g.Pa), j = (int)(idx % (uint32_t)g.Pa);
-        const double Td = T[s];
-        if (!(Td > 1e-10)) continue;  // emc_atomic.cu:190
-        const float val = U[(size_t)(s - s0) * g.Pp + j];
-        if (!(val >= 0.0f)) continue;  // emc_atomic.cu:56
-        const float Ts = (float)Td;
         float m[9];
         load_rot(rotm, s, m);
         int bx, by, bz;
@@ -220,13 +240,16 @@ hipError_t build_bins(emc_ctx *c, std::string &err)
         if (e == hipSuccess) e = r;
     };
     if (c->bin_vals) (void)hipFree(c->bin_vals);
+    if (c->bin_perm) (void)hipFree(c->bin_perm);
     if (c->bin_units) (void)hipFree(c->bin_units);
     c->bin_vals = nullptr;
+    c->bin_perm = nullptr;
     c->bin_units = nullptr;
     chk(hipMalloc((void **)&kin, sizeof(uint32_t) * n));
     chk(hipMalloc((void **)&vin, sizeof(uint32_t) * n));
     chk(hipMalloc((void **)&kout, sizeof(uint32_t) * n));
     chk(hipMalloc((void **)&c->bin_vals, sizeof(uint32_t) * n));
+    chk(hipMalloc((void **)&c->bin_perm, sizeof(uint32_t) * n));
     chk(hipMalloc((void **)&cnt, sizeof(int32_t) * 2 * (size_t)nkeys));
     if (e == hipSuccess) {
         const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
@@ -265,8 +288,8 @@ hipError_t build_bins(emc_ctx *c, std::string &err)
         chk(hipMalloc((void **)&c->bin_units, sizeof(int32_t) * units.size()));
         if (e == hipSuccess)
             chk(hipMemcpy(c->bin_units, units.data(), sizeof(int32_t) * units.size(), hipMemcpyHostToDevice));
-        // within each unit, order the items by base voxel (stable: then by (s, j)), so the
-        // insertion can sum runs of one voxel in registers
+        // within each unit, list the items' positions by base voxel (stable: then by (s, j)), so
+        // the insertion can sum runs of one voxel in registers; bin_vals keeps the (s, j) order
         const uint32_t n_units = (uint32_t)(units.size() / 3);
         const uint64_t nkeys2 = (uint64_t)n_units * kBrick * kBrick * kBrick;
         int end_bit2 = 1;
@@ -280,10 +303,14 @@ hipError_t build_bins(emc_ctx *c, std::string &err)
                                g, kin);
             chk(hipGetLastError());
         }
-        if (e == hipSuccess) chk(hipMemcpyAsync(vin, c->bin_vals, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+        if (e == hipSuccess) {
+            const int blocks = (int)std::min<int64_t>((n + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_iota, dim3(blocks), dim3(256), 0, st, vin, n);
+            chk(hipGetLastError());
+        }
         size_t tmp2 = 0;
         if (e == hipSuccess)
-            chk(rocprim::radix_sort_pairs(nullptr, tmp2, kin, kout, vin, c->bin_vals, (size_t)n, 0, end_bit2, st));
+            chk(rocprim::radix_sort_pairs(nullptr, tmp2, kin, kout, vin, c->bin_perm, (size_t)n, 0, end_bit2, st));
         if (e == hipSuccess && tmp2 > tmp_bytes) {
             (void)hipFree(tmp);
             tmp = nullptr;
@@ -291,7 +318,7 @@ hipError_t build_bins(emc_ctx *c, std::string &err)
             tmp_bytes = tmp2;
         }
         if (e == hipSuccess)
-            chk(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, c->bin_vals, (size_t)n, 0, end_bit2, st));
+            chk(rocprim::radix_sort_pairs(tmp, tmp_bytes, kin, kout, vin, c->bin_perm, (size_t)n, 0, end_bit2, st));
         if (e == hipSuccess) chk(hipStreamSynchronize(st));
     }
     (void)hipFree(kin);
@@ -313,10 +340,11 @@ hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st
     const BinGeom g = geom_of(c);
     if (c->opt.interp == EMC_INTERP_NEAREST)
         hipLaunchKernelGGL(k_insert_binned<EMC_INTERP_NEAREST>, dim3(u1 - u0), dim3(256), 0, st, c->bin_units + 3 * u0,
-                           c->bin_vals, c->U, s0, c->T, c->rotm, c->axyz, g, c->Mnum, c->Wt);
+                           c->bin_vals, c->bin_perm, c->U, s0, c->T, c->rotm, c->axyz, g, c->Mnum, c->Wt);
     else
         hipLaunchKernelGGL(k_insert_binned<EMC_INTERP_TRILINEAR>, dim3(u1 - u0), dim3(256), 0, st,
-                           c->bin_units + 3 * u0, c->bin_vals, c->U, s0, c->T, c->rotm, c->axyz, g, c->Mnum, c->Wt);
+                           c->bin_units + 3 * u0, c->bin_vals, c->bin_perm, c->U, s0, c->T, c->rotm, c->axyz, g, c->Mnum,
+                           c->Wt);
     return hipGetLastError();
 }
 

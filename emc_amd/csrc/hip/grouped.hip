@@ -9,11 +9,11 @@
 // cell_rot[rbeg[c] .. rbeg[c+1]) (nr_c of them) and the "slots" sbeg[c] .. sbeg[c+1) (ns_c
 // patterns that selected c; slot_pat gives the pattern).  Per cell, logits and responsibilities
 // are a row-major block [nr_c][nsp_c] (nsp_c = ns_c rounded up to 32, padding columns hold P = 0)
-// at element offset boff[c] of c->R (logits) and c->Ph / c->Pl (P hi / lo).  The M-step reads the
-// cell's counts transposed and gathered, KTg block [Pp][nsp_c] at element gbeg[c] * Pp.
+// at element offset boff[c] of c->R (logits) and c->Ph / c->Pl (P hi / lo).
 //
 //   logL   : block(c)[r][n] = sum_j logW~[rot(r), j] K[pat(n), j] + bias[rot(r)]    (rows gathered)
-//   M-step : U[rot(r), j]   = sum_n P(c)[r][n] KTg(c)[j][n]
+//   M-step : U[rot(r), j]   = sum_n P(c)[r][n] K[pat(n), j]      (count rows gathered, transposed
+//                                                                  into LDS while staged)
 // Tiles are 128 x 128, 256 threads = 4 wavefronts each 64 x 64 (2 x 2 v_mfma_f32_32x32x16_bf16),
 // BK = 32 with two LDS stages (rows padded to 40 BF16), workgroups renumbered per XCD.
 #include <algorithm>
@@ -32,7 +32,7 @@ constexpr int TM = 128, TN = 128, TK = 32, TLD = TK + 8;
 
 struct GTab {
     const int32_t *cell_rot, *rbeg, *cell_of;           // partition of the fine rotations
-    const int32_t *slot_pat, *sbeg, *nsp, *gbeg;         // slots per cell
+    const int32_t *slot_pat, *sbeg, *nsp;                // slots per cell
     const int32_t *pbeg, *pslot;                         // slots per pattern
     const int64_t *boff;                                 // block offsets
     const int4 *tiles;                                   // (cell, m0, n0, -)
@@ -62,7 +62,9 @@ enum { G_LOGL = 0, G_MSTEP = 1 };
 
 // MODE G_LOGL : A = NA log-slice terms [S][lda] (rows gathered by cell_rot), B = counts [I][ldb]
 //               (rows gathered by slot_pat), k extent Kd = Pp.
-// MODE G_MSTEP: A = P hi, lo blocks (NA = 2), B = KTg block, k extent nsp_c.
+// MODE G_MSTEP: A = P hi, lo blocks (NA = 2), B = counts [I][ldb] read as B(j, n) = K[pat(n)][j]
+//               (each thread stages 8 pixels of 2 slots and writes them as slot pairs), k extent
+//               nsp_c.
 template <int MODE, int NA, bool BU8>
 __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
                                                  const bf16_t *__restrict__ A2, int64_t lda,
@@ -109,16 +111,18 @@ __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, 
         }
     } else {
         Kd = nsp;
-        const int64_t kt0 = (int64_t)g.gbeg[c] * Pp;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int64_t arow = boff + (int64_t)min(m0 + lrow + 64 * h, nr - 1) * nsp + lcol;
             ga[0][h] = A0 + arow;
             if (NA > 1) ga[1][h] = A1 + arow;
-            const int64_t brow = kt0 + (int64_t)(n0 + lrow + 64 * h) * nsp + lcol;
-            gb[h] = reinterpret_cast<const unsigned char *>(Bv) + brow * (BU8 ? 1 : 2);
         }
     }
+    // M-step B staging: slots 2 kq, 2 kq + 1 of the k block, pixels n0 + 8 ch .. + 7; padding
+    // slots read the cell's last pattern (their P is 0)
+    const int kq = tid & 15, ch = tid >> 4;
+    const int32_t *spat = g.slot_pat + sb;
+    const unsigned char *kb = reinterpret_cast<const unsigned char *>(Bv) + (int64_t)(n0 + 8 * ch) * (BU8 ? 1 : 2);
 
     u32x4 ra[NA][2], rb16[2];
     u32x2 rb8[2];
@@ -129,8 +133,15 @@ __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, 
             for (int h = 0; h < 2; ++h) ra[a][h] = *reinterpret_cast<const u32x4 *>(ga[a][h] + k0);
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            if (BU8) rb8[h] = *reinterpret_cast<const u32x2 *>(gb[h] + k0);
-            else rb16[h] = *reinterpret_cast<const u32x4 *>(gb[h] + (size_t)k0 * 2);
+            if (MODE == G_LOGL) {
+                if (BU8) rb8[h] = *reinterpret_cast<const u32x2 *>(gb[h] + k0);
+                else rb16[h] = *reinterpret_cast<const u32x4 *>(gb[h] + (size_t)k0 * 2);
+            } else {
+                const int64_t row = spat[min(k0 + 2 * kq + h, ns - 1)];
+                const unsigned char *src = kb + row * ldb * (BU8 ? 1 : 2);
+                if (BU8) rb8[h] = *reinterpret_cast<const u32x2 *>(src);
+                else rb16[h] = *reinterpret_cast<const u32x4 *>(src);
+            }
         }
     };
     auto sstore = [&](int stage) {
@@ -140,10 +151,20 @@ __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, 
 #pragma unroll
             for (int h = 0; h < 2; ++h)
                 *reinterpret_cast<u32x4 *>(S + ((size_t)a * TM + lrow + 64 * h) * TLD + lcol) = ra[a][h];
+        if (MODE == G_LOGL) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
-            *reinterpret_cast<u32x4 *>(S + ((size_t)NA * TM + lrow + 64 * h) * TLD + lcol) =
-                BU8 ? widen_u8x8(rb8[h]) : rb16[h];
+            for (int h = 0; h < 2; ++h)
+                *reinterpret_cast<u32x4 *>(S + ((size_t)NA * TM + lrow + 64 * h) * TLD + lcol) =
+                    BU8 ? widen_u8x8(rb8[h]) : rb16[h];
+        } else {
+            const u32x4 w0 = BU8 ? widen_u8x8(rb8[0]) : rb16[0], w1 = BU8 ? widen_u8x8(rb8[1]) : rb16[1];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {  // pixel 8 ch + q: (slot 2 kq, slot 2 kq + 1) as one word
+                const uint32_t lo = (w0[q >> 1] >> (16 * (q & 1))) & 0xffffu;
+                const uint32_t hi = (w1[q >> 1] >> (16 * (q & 1))) & 0xffffu;
+                *reinterpret_cast<uint32_t *>(S + ((size_t)NA * TM + 8 * ch + q) * TLD + 2 * kq) = lo | (hi << 16);
+            }
+        }
     };
 
     f32x16 acc[2][2];
@@ -203,52 +224,6 @@ __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, 
                 }
             }
         }
-}
-
-// KTg block of cell c, padded slots g0 .. g0+31: KTg[gbeg_c*Pp + j*nsp_c + g] = K[pat(g)][j]
-// (0 for padding slots).  Per 128-pixel chunk: 16-byte loads of the 32 pattern rows into LDS,
-// then each pixel's 32 slot values leave as 16-byte stores (two threads per pixel).
-template <typename T>
-__global__ __launch_bounds__(256) void k_build_ktg(const T *__restrict__ K, int64_t ldk, int Pp, GTab g,
-                                                   T *__restrict__ ktg)
-{
-    constexpr int ES = (int)sizeof(T);
-    constexpr int PW = 128;                 // pixels per chunk
-    constexpr int RC = PW * ES / 16;        // 16-byte chunks per row segment
-    constexpr int LDT = PW + 16 / ES;       // LDS row pitch (elements), 16-byte aligned
-    __shared__ __attribute__((aligned(16))) T tile[32][LDT];
-    const int4 td = g.tiles[blockIdx.x];
-    const int c = td.x, g0 = td.y;
-    const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
-    const int64_t out0 = (int64_t)g.gbeg[c] * Pp;
-    const int tid = threadIdx.x;
-    const int p = tid >> 1, half = tid & 1;  // output: pixel p of the chunk, slots 16*half .. +15
-    for (int p0 = 0; p0 < Pp; p0 += PW) {    // Pp is a multiple of 128
-        for (int q = tid; q < 32 * RC; q += 256) {
-            const int r = q / RC, cc = q % RC, s = g0 + r;
-            u32x4 v = {0u, 0u, 0u, 0u};
-            if (s < ns)
-                v = *reinterpret_cast<const u32x4 *>(K + (int64_t)g.slot_pat[sb + s] * ldk + p0 + cc * (16 / ES));
-            *reinterpret_cast<u32x4 *>(&tile[r][cc * (16 / ES)]) = v;
-        }
-        __syncthreads();
-        T *dst = ktg + out0 + (int64_t)(p0 + p) * nsp + g0 + 16 * half;
-        constexpr int PER = 4 / ES;  // values per 32-bit word
-#pragma unroll
-        for (int k = 0; k < ES; ++k) {  // 16 values = ES stores of 16 bytes
-            u32x4 o;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int b = 0; b < PER; ++b)
-                    word |= (uint32_t)tile[16 * half + k * 4 * PER + w * PER + b][p] << (8 * ES * b);
-                o[w] = word;
-            }
-            reinterpret_cast<u32x4 *>(dst)[k] = o;
-        }
-        __syncthreads();
-    }
 }
 
 // Softmax partials: one wavefront per 64 slots of a cell (tiles (cell, s0)); online (max, FP64
@@ -344,6 +319,77 @@ __global__ __launch_bounds__(256) void k_gsm_apply(const float *__restrict__ Lb,
     if (lane == 0) T[rot] = acc;
 }
 
+// The kMaxRecall cells of largest score seen so far, sorted by score (descending; on equal
+// scores the cell offered first stays first).  Static indexing only, so the arrays stay in
+// registers.
+struct TopCells {
+    float v[kMaxRecall];
+    int c[kMaxRecall];
+    __device__ void init()
+    {
+#pragma unroll
+        for (int q = 0; q < kMaxRecall; ++q) {
+            v[q] = -INFINITY;
+            c[q] = -1;
+        }
+    }
+    __device__ void offer(float x, int cell)
+    {
+#pragma unroll
+        for (int q = 0; q < kMaxRecall; ++q) {
+            if (x > v[q]) {
+                const float tv = v[q];
+                const int tc = c[q];
+                v[q] = x;
+                c[q] = cell;
+                x = tv;
+                cell = tc;
+            }
+        }
+    }
+    __device__ void store(int32_t *out, int R) const
+    {
+#pragma unroll
+        for (int q = 0; q < kMaxRecall; ++q)
+            if (q < R) out[q] = c[q];
+    }
+};
+
+// Recall after a full search: the cells of largest max logit, Mx [C][ld] (-inf for an empty
+// cell, never recalled).  One thread per pattern (reads coalesced across patterns).
+__global__ __launch_bounds__(256) void k_recall_dense(const float *__restrict__ Mx, int64_t ld, int I, int C, int R,
+                                                      int32_t *__restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= I) return;
+    TopCells t;
+    t.init();
+    for (int c = 0; c < C; ++c) t.offer(Mx[(size_t)c * ld + i], c);
+    t.store(out + (size_t)i * R, R);
+}
+
+// Recall after a proposal iteration: the searched cells of largest max logit (the slot partial
+// maximum pm).  The cell of a slot is found by binary search in sbeg.
+__global__ __launch_bounds__(256) void k_recall_slots(const float *__restrict__ pm, GTab g, int I, int C, int R,
+                                                      int32_t *__restrict__ out)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= I) return;
+    TopCells t;
+    t.init();
+    for (int q = g.pbeg[i]; q < g.pbeg[i + 1]; ++q) {
+        const int sl = g.pslot[q];
+        int lo = 0, hi = C;  // largest c with sbeg[c] <= sl
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (g.sbeg[mid] <= sl) lo = mid;
+            else hi = mid;
+        }
+        t.offer(pm[sl], lo);
+    }
+    t.store(out + (size_t)i * R, R);
+}
+
 GTab tables(const emc_ctx *c, const int4 *tiles)
 {
     const Proposal *p = c->prop;
@@ -354,7 +400,6 @@ GTab tables(const emc_ctx *c, const int4 *tiles)
     g.slot_pat = p->lists + p->off_slot_pat;
     g.sbeg = p->lists + p->off_sbeg;
     g.nsp = p->lists + p->off_nsp;
-    g.gbeg = p->lists + p->off_gbeg;
     g.pbeg = p->lists + p->off_pbeg;
     g.pslot = p->lists + p->off_pslot;
     g.boff = p->boff;
@@ -410,17 +455,24 @@ hipError_t launch_grouped_softmax(const emc_ctx *c, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_build_ktg(const emc_ctx *c, hipStream_t st)
+hipError_t launch_recall_grouped(const emc_ctx *c, hipStream_t st)
 {
     const Proposal *p = c->prop;
-    if (p->n_tiles_kt == 0) return hipSuccess;
-    const GTab g = tables(c, tile_list(p, p->off_tile_kt));
-    if (c->K8)
-        hipLaunchKernelGGL(k_build_ktg<uint8_t>, dim3(p->n_tiles_kt), dim3(256), 0, st, c->K8, (int64_t)c->Pp, c->Pp, g,
-                           reinterpret_cast<uint8_t *>(p->ktg));
-    else
-        hipLaunchKernelGGL(k_build_ktg<uint16_t>, dim3(p->n_tiles_kt), dim3(256), 0, st, c->K16, (int64_t)c->Pp, c->Pp,
-                           g, reinterpret_cast<uint16_t *>(p->ktg));
+    const int R = p->o.recall;
+    if (R < 1 || R > kMaxRecall) return R == 0 ? hipSuccess : hipErrorInvalidValue;
+    const GTab g = tables(c, nullptr);
+    hipLaunchKernelGGL(k_recall_slots, dim3((c->I + 255) / 256), dim3(256), 0, st, p->slot_pm, g, c->I, p->C, R,
+                       p->recall);
+    return hipGetLastError();
+}
+
+hipError_t launch_recall_dense(const emc_ctx *c, hipStream_t st)
+{
+    const Proposal *p = c->prop;
+    const int R = p->o.recall;
+    if (R < 1 || R > kMaxRecall) return R == 0 ? hipSuccess : hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_recall_dense, dim3((c->I + 255) / 256), dim3(256), 0, st, p->Mx, (int64_t)c->Ip, c->I, p->C, R,
+                       p->recall);
     return hipGetLastError();
 }
 
@@ -431,10 +483,10 @@ hipError_t launch_grouped_mstep(const emc_ctx *c, hipStream_t st)
     const GTab g = tables(c, tile_list(p, p->off_tile_m));
     const dim3 grid(p->n_tiles_m), blk(256);
     if (c->K8)
-        hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, true>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, p->ktg, 0, c->Pp,
+        hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, true>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, c->K8, c->Pp, c->Pp,
                            g, nullptr, c->U);
     else
-        hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, false>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, p->ktg, 0, c->Pp,
+        hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, false>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, c->K16, c->Pp, c->Pp,
                            g, nullptr, c->U);
     return hipGetLastError();
 }

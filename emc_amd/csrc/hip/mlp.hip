@@ -202,14 +202,34 @@ __global__ __launch_bounds__(256) void k_accum_f64(const double *__restrict__ x,
     if (threadIdx.x == 0) *acc += red[0] + red[1] + red[2] + red[3];
 }
 
-// Column sums of a BF16 matrix [rows][ld] -> g[n] (bias gradients).
-__global__ void k_colsum(const bf16_t *__restrict__ d, int rows, int ld, int n_valid, float *__restrict__ g)
+// Column sums of a BF16 matrix [rows][ld] -> g[n] (bias gradients).  A block covers 32 columns
+// with 8 row groups (rows g, g + 8, ...; eight loads in flight per thread), combined in LDS in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_colsum(const bf16_t *__restrict__ d, int rows, int ld, int n_valid,
+                                                float *__restrict__ g)
 {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= n_valid) return;
+    __shared__ float part[8][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5, n = blockIdx.x * 32 + tx;
     float acc = 0.0f;
-    for (int b = 0; b < rows; ++b) acc += bf2f(d[(size_t)b * ld + n]);
-    g[n] = acc;
+    if (n < n_valid) {
+        int b = ty;
+        for (; b + 56 < rows; b += 64) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = bf2f(d[(size_t)(b + 8 * u) * ld + n]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u];
+        }
+        for (; b < rows; b += 8) acc += bf2f(d[(size_t)b * ld + n]);
+    }
+    part[ty][tx] = acc;
+    __syncthreads();
+    if (ty == 0 && n < n_valid) {
+        float s = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) s += part[r][tx];
+        g[n] = s;
+    }
 }
 
 // Adam (Kingma & Ba 2015) on FP32 masters, BF16 copy refreshed.  c1, c2 = bias corrections.
@@ -282,26 +302,53 @@ __global__ __launch_bounds__(64) void k_topk(const float *__restrict__ p, int Cp
     }
 }
 
+// Predicted mass of the cells a proposal iteration searches for each pattern: the top-k cells'
+// (already in mass[i]) plus the recalled cells' not among them.
+__global__ void k_union_mass(const float *__restrict__ p, int Cp, int I, const int32_t *__restrict__ cells, int k,
+                             const int32_t *__restrict__ recall, int R, float *__restrict__ mass)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= I) return;
+    const int32_t *ci = cells + (size_t)i * k, *ri = recall + (size_t)i * R;
+    float a = mass[i];
+    for (int q = 0; q < R; ++q) {
+        const int cr = ri[q];
+        bool seen = cr < 0;
+        for (int u = 0; u < k; ++u) seen |= ci[u] == cr;
+        for (int u = 0; u < q; ++u) seen |= ri[u] == cr;
+        if (!seen) a += p[(size_t)i * Cp + cr];
+    }
+    mass[i] = a;
+}
+
 // Cell sums of the full responsibilities: Tt[c][i] = sum over rotations s of cell c of
-// (Ph + Pl)[s][i].  Grid (pattern blocks, cells); rows of P are read coalesced.
+// (Ph + Pl)[s][i]; with LOGITS also the cell's largest logit Mx[c][i] = max_s Lg[s][i] (-inf for
+// an empty cell).  Grid (pattern blocks, cells); rows are read coalesced.
+template <bool LOGITS>
 __global__ __launch_bounds__(256) void k_cell_targets(const bf16_t *__restrict__ Ph, const bf16_t *__restrict__ Pl,
-                                                      int64_t ld, int I, const int32_t *__restrict__ cell_rot,
-                                                      const int32_t *__restrict__ rbeg, float *__restrict__ Tt)
+                                                      const float *__restrict__ Lg, int64_t ld, int I,
+                                                      const int32_t *__restrict__ cell_rot,
+                                                      const int32_t *__restrict__ rbeg, float *__restrict__ Tt,
+                                                      float *__restrict__ Mx)
 {
     const int i = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
     if (i >= I) return;
-    float acc = 0.0f;
+    float acc = 0.0f, mx = -INFINITY;
     for (int g = rbeg[c]; g < rbeg[c + 1]; ++g) {
         const size_t o = (size_t)cell_rot[g] * ld + i;
         acc += bf2f(Ph[o]) + bf2f(Pl[o]);
+        if (LOGITS) mx = fmaxf(mx, Lg[o]);
     }
     Tt[(size_t)c * ld + i] = acc;
+    if (LOGITS) Mx[(size_t)c * ld + i] = mx;
 }
 
-// Responsibility mass inside the predicted cells of each pattern (all cells when the predicted
-// mass is below min_mass, as a proposal iteration would search them).
+// Responsibility mass inside the cells a proposal iteration would search for each pattern: the
+// predicted top k and the R recalled ones (-1 = none), each cell counted once; all cells when the
+// predicted mass is below min_mass.
 __global__ void k_captured(const float *__restrict__ Tt, int64_t ld, int I, int C, const int32_t *__restrict__ cells,
-                           const float *__restrict__ mass, int k, float min_mass, double *__restrict__ cap)
+                           const float *__restrict__ mass, int k, float min_mass, const int32_t *__restrict__ recall,
+                           int R, double *__restrict__ cap)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= I) return;
@@ -309,7 +356,16 @@ __global__ void k_captured(const float *__restrict__ Tt, int64_t ld, int I, int 
     if (mass[i] < min_mass) {
         for (int c = 0; c < C; ++c) a += Tt[(size_t)c * ld + i];
     } else {
-        for (int q = 0; q < k; ++q) a += Tt[(size_t)cells[(size_t)i * k + q] * ld + i];
+        const int32_t *ci = cells + (size_t)i * k;
+        for (int q = 0; q < k; ++q) a += Tt[(size_t)ci[q] * ld + i];
+        const int32_t *ri = recall + (size_t)i * R;
+        for (int q = 0; q < R; ++q) {
+            const int cr = ri[q];
+            bool seen = cr < 0;
+            for (int u = 0; u < k; ++u) seen |= ci[u] == cr;
+            for (int u = 0; u < q; ++u) seen |= ri[u] == cr;
+            if (!seen) a += Tt[(size_t)cr * ld + i];
+        }
     }
     cap[i] = a;
 }
@@ -482,7 +538,7 @@ hipError_t mlp_accum(const double *x, int n, double *acc, hipStream_t st)
 
 hipError_t mlp_colsum(const bf16_t *d, int rows, int ld, int n_valid, float *g, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_colsum, dim3((n_valid + 255) / 256), dim3(256), 0, st, d, rows, ld, n_valid, g);
+    hipLaunchKernelGGL(k_colsum, dim3((n_valid + 31) / 32), dim3(256), 0, st, d, rows, ld, n_valid, g);
     return hipGetLastError();
 }
 
@@ -511,11 +567,25 @@ hipError_t mlp_topk(const float *p, int Cp, int C, int rows, int k, int32_t *cel
     return hipGetLastError();
 }
 
-hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, hipStream_t st)
+hipError_t mlp_union_mass(const emc_ctx *c, hipStream_t st)
 {
     const Proposal *pr = c->prop;
-    hipLaunchKernelGGL(k_cell_targets, dim3((c->I + 255) / 256, pr->C), dim3(256), 0, st, c->Ph, c->Pl,
-                       (int64_t)c->Ip, c->I, pr->cell_rot, pr->rbeg, Tt);
+    if (pr->o.recall == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_union_mass, dim3((c->I + 255) / 256), dim3(256), 0, st, pr->prob, pr->Cp, c->I, pr->topc,
+                       pr->o.top_k, pr->recall, pr->o.recall, pr->pmass);
+    return hipGetLastError();
+}
+
+hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, float *Mx, hipStream_t st)
+{
+    const Proposal *pr = c->prop;
+    const dim3 grid((c->I + 255) / 256, pr->C);
+    if (Mx)
+        hipLaunchKernelGGL(k_cell_targets<true>, grid, dim3(256), 0, st, c->Ph, c->Pl, c->R, (int64_t)c->Ip, c->I,
+                           pr->cell_rot, pr->rbeg, Tt, Mx);
+    else
+        hipLaunchKernelGGL(k_cell_targets<false>, grid, dim3(256), 0, st, c->Ph, c->Pl, nullptr, (int64_t)c->Ip, c->I,
+                           pr->cell_rot, pr->rbeg, Tt, nullptr);
     return hipGetLastError();
 }
 
@@ -523,7 +593,7 @@ hipError_t mlp_captured(const emc_ctx *c, double *cap, hipStream_t st)
 {
     const Proposal *pr = c->prop;
     hipLaunchKernelGGL(k_captured, dim3((c->I + 255) / 256), dim3(256), 0, st, pr->Tt, (int64_t)c->Ip, c->I, pr->C,
-                       pr->topc, pr->pmass, pr->o.top_k, pr->o.min_mass, cap);
+                       pr->topc, pr->pmass, pr->o.top_k, pr->o.min_mass, pr->recall, pr->o.recall, cap);
     return hipGetLastError();
 }
 

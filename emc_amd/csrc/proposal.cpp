@@ -177,34 +177,68 @@ void proposal_free(Proposal *p)
     pfree(p->mu); pfree(p->istd); pfree(p->X); pfree(p->raw);
     pfree(p->pre1); pfree(p->pre2); pfree(p->z); pfree(p->prob);
     pfree(p->h1); pfree(p->h2); pfree(p->dz); pfree(p->d1); pfree(p->d2); pfree(p->loss_part);
-    pfree(p->Tt); pfree(p->Xb); pfree(p->tb); pfree(p->bidx); pfree(p->syn_counts); pfree(p->Xs); pfree(p->syn_rot);
-    pfree(p->topc); pfree(p->pmass); pfree(p->cap);
+    pfree(p->Tt); pfree(p->Mx); pfree(p->Xb); pfree(p->tb); pfree(p->bidx); pfree(p->syn_counts); pfree(p->Xs); pfree(p->syn_rot);
+    pfree(p->topc); pfree(p->pmass); pfree(p->cap); pfree(p->recall);
+    phfree(p->topc_h); phfree(p->pmass_h); phfree(p->recall_h);
+    if (p->ev_pred) (void)hipEventDestroy(p->ev_pred);
+    p->ev_pred = nullptr;
     pfree(p->lists); phfree(p->lists_h); pfree(p->boff); phfree(p->boff_h);
-    pfree(p->ktg); pfree(p->slot_ps); pfree(p->slot_pm); pfree(p->slot_lse);
+    pfree(p->slot_ps); pfree(p->slot_pm); pfree(p->slot_lse);
+}
+
+emc_status proposal_predict_enqueue(emc_ctx *c)
+{
+    Proposal *p = c->prop;
+    hipStream_t st = c->stream;
+    p->pred_valid = false;
+    P_TRY(forward(c, p->X, c->Ip, false));
+    P_HIP(c, mlp_softmax_ce(p->z, p->Cp, p->C, c->I, c->Ip, nullptr, 0.0f, p->prob, nullptr, nullptr, st));
+    P_HIP(c, mlp_topk(p->prob, p->Cp, p->C, c->I, p->o.top_k, p->topc, p->pmass, st));
+    P_HIP(c, mlp_union_mass(c, st));  // pmass: predicted mass of every cell the iteration would search
+    P_HIP(c, hipMemcpyAsync(p->topc_h, p->topc, sizeof(int32_t) * (size_t)c->I * p->o.top_k, hipMemcpyDeviceToHost, st));
+    P_HIP(c, hipMemcpyAsync(p->pmass_h, p->pmass, sizeof(float) * c->I, hipMemcpyDeviceToHost, st));
+    if (p->o.recall > 0)
+        P_HIP(c, hipMemcpyAsync(p->recall_h, p->recall, sizeof(int32_t) * (size_t)c->I * p->o.recall,
+                                hipMemcpyDeviceToHost, st));
+    P_HIP(c, hipEventRecord(p->ev_pred, st));
+    return EMC_OK;
+}
+
+emc_status proposal_predict_wait(emc_ctx *c)
+{
+    P_HIP(c, hipEventSynchronize(c->prop->ev_pred));
+    c->prop->pred_valid = true;
+    return EMC_OK;
 }
 
 emc_status proposal_predict(emc_ctx *c)
 {
-    Proposal *p = c->prop;
-    hipStream_t st = c->stream;
-    P_TRY(forward(c, p->X, c->Ip, false));
-    P_HIP(c, mlp_softmax_ce(p->z, p->Cp, p->C, c->I, c->Ip, nullptr, 0.0f, p->prob, nullptr, nullptr, st));
-    P_HIP(c, mlp_topk(p->prob, p->Cp, p->C, c->I, p->o.top_k, p->topc, p->pmass, st));
-    p->topc_h.resize((size_t)c->I * p->o.top_k);
-    p->pmass_h.resize(c->I);
-    P_HIP(c, hipMemcpyAsync(p->topc_h.data(), p->topc, sizeof(int32_t) * p->topc_h.size(), hipMemcpyDeviceToHost, st));
-    P_HIP(c, hipMemcpyAsync(p->pmass_h.data(), p->pmass, sizeof(float) * c->I, hipMemcpyDeviceToHost, st));
-    P_HIP(c, hipStreamSynchronize(st));
-    p->pred_valid = true;
-    return EMC_OK;
+    P_TRY(proposal_predict_enqueue(c));
+    return proposal_predict_wait(c);
 }
 
 emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
 {
     Proposal *p = c->prop;
-    const int I = c->I, C = p->C, k = p->o.top_k, S = c->S, Pp = c->Pp;
+    const int I = c->I, C = p->C, k = p->o.top_k, R = p->o.recall, S = c->S, Pp = c->Pp;
     const std::vector<int32_t> &rbeg = p->rbeg_h;
     auto nr = [&](int cc) { return rbeg[cc + 1] - rbeg[cc]; };
+
+    // cells searched for pattern i (unless it falls back): the predicted top k, then the
+    // recalled cells not among them
+    auto cells_of = [&](int i, int32_t *out) {
+        int n = 0;
+        for (int q = 0; q < k; ++q) out[n++] = p->topc_h[(size_t)i * k + q];
+        for (int q = 0; q < R; ++q) {
+            const int32_t cr = p->recall_h[(size_t)i * R + q];
+            if (cr < 0 || cr >= C) continue;
+            bool seen = false;
+            for (int u = 0; u < n; ++u) seen |= out[u] == cr;
+            if (!seen) out[n++] = cr;
+        }
+        return n;
+    };
+    std::vector<int32_t> cs(k + R);
 
     // patterns searched over every cell: low predicted mass, or no non-empty cell among the top k
     std::vector<uint8_t> fb(I, 0);
@@ -224,40 +258,36 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
         if (fb[i]) {
             for (int cc = 0; cc < C; ++cc) cnt[cc] += nr(cc) > 0;
         } else {
-            for (int q = 0; q < k; ++q) {
-                const int cc = p->topc_h[(size_t)i * k + q];
-                cnt[cc] += nr(cc) > 0;
-            }
+            const int n = cells_of(i, cs.data());
+            for (int q = 0; q < n; ++q) cnt[cs[q]] += nr(cs[q]) > 0;
         }
     }
-    std::vector<int64_t> sbeg(C + 1, 0), gbeg(C + 1, 0);
+    std::vector<int64_t> sbeg(C + 1, 0);
     std::vector<int32_t> nsp(C, 0);
     int64_t blk = 0;
     double pairs = 0.0;
     for (int cc = 0; cc < C; ++cc) {
         sbeg[cc + 1] = sbeg[cc] + cnt[cc];
         nsp[cc] = cnt[cc] ? (int32_t)rup(cnt[cc], 32) : 0;
-        gbeg[cc + 1] = gbeg[cc] + nsp[cc];
         p->boff_h[cc] = blk;
         blk += (int64_t)nr(cc) * nsp[cc];
         pairs += (double)nr(cc) * (double)cnt[cc];
     }
-    const int64_t n_slots = sbeg[C], n_g = gbeg[C];
-    if (n_g > INT32_MAX / 2 || blk > (int64_t)c->Sp * c->Ip) return pfail(c, EMC_E_UNSUPPORTED, "proposal: work lists too large");
+    const int64_t n_slots = sbeg[C];
+    if (n_slots > INT32_MAX / 2 || blk > (int64_t)c->Sp * c->Ip)
+        return pfail(c, EMC_E_UNSUPPORTED, "proposal: work lists too large");
     p->n_slots = n_slots;
-    p->n_gslots = n_g;
     p->n_blk = blk;
     p->work_fraction = pairs / ((double)S * (double)I);
 
     // tiles
-    int64_t n_te = 0, n_tm = 0, n_ts = 0, n_tk = 0;
+    int64_t n_te = 0, n_tm = 0, n_ts = 0;
     for (int cc = 0; cc < C; ++cc) {
         if (!cnt[cc]) continue;
         const int64_t mt = (nr(cc) + 127) / 128;
         n_te += mt * ((cnt[cc] + 127) / 128);
         n_tm += mt * (Pp / 128);
         n_ts += (cnt[cc] + 63) / 64;
-        n_tk += nsp[cc] / 32;
     }
     // packed int32 lists, each section starting on a 16-byte boundary
     int64_t off = 0;
@@ -269,13 +299,11 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
     p->off_slot_pat = sec(n_slots);
     p->off_sbeg = sec(C + 1);
     p->off_nsp = sec(C);
-    p->off_gbeg = sec(C + 1);
     p->off_pbeg = sec(I + 1);
     p->off_pslot = sec(n_slots);
     p->off_tile_e = sec(4 * n_te);
     p->off_tile_m = sec(4 * n_tm);
     p->off_tile_sm = sec(4 * n_ts);
-    p->off_tile_kt = sec(4 * n_tk);
     const int64_t need = off;
     if (need > p->list_cap) {
         pfree(p->lists);
@@ -287,10 +315,7 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
     }
     int32_t *L = p->lists_h;
     int32_t *slot_pat = L + p->off_slot_pat, *pbeg = L + p->off_pbeg, *pslot = L + p->off_pslot;
-    for (int cc = 0; cc <= C; ++cc) {
-        L[p->off_sbeg + cc] = (int32_t)sbeg[cc];
-        L[p->off_gbeg + cc] = (int32_t)gbeg[cc];
-    }
+    for (int cc = 0; cc <= C; ++cc) L[p->off_sbeg + cc] = (int32_t)sbeg[cc];
     for (int cc = 0; cc < C; ++cc) L[p->off_nsp + cc] = nsp[cc];
     std::vector<int64_t> cur(sbeg.begin(), sbeg.end() - 1);
     int64_t q_out = 0;
@@ -305,11 +330,12 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
         if (fb[i]) {
             for (int cc = 0; cc < C; ++cc) take(cc);
         } else {
-            for (int q = 0; q < k; ++q) take(p->topc_h[(size_t)i * k + q]);
+            const int n = cells_of(i, cs.data());
+            for (int q = 0; q < n; ++q) take(cs[q]);
         }
         pbeg[i + 1] = (int32_t)q_out;
     }
-    int32_t *te = L + p->off_tile_e, *tm = L + p->off_tile_m, *ts = L + p->off_tile_sm, *tk = L + p->off_tile_kt;
+    int32_t *te = L + p->off_tile_e, *tm = L + p->off_tile_m, *ts = L + p->off_tile_sm;
     for (int cc = 0; cc < C; ++cc) {
         if (!cnt[cc]) continue;
         for (int m0 = 0; m0 < nr(cc); m0 += 128) {
@@ -326,27 +352,12 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
             ts[0] = cc; ts[1] = s0; ts[2] = 0; ts[3] = 0;
             ts += 4;
         }
-        for (int g0 = 0; g0 < nsp[cc]; g0 += 32) {
-            tk[0] = cc; tk[1] = g0; tk[2] = 0; tk[3] = 0;
-            tk += 4;
-        }
     }
     p->n_tiles_e = (int)n_te;
     p->n_tiles_m = (int)n_tm;
     p->n_tiles_sm = (int)n_ts;
-    p->n_tiles_kt = (int)n_tk;
 
     // device scratch sized to this iteration's lists
-    const int64_t esz = c->K8 ? 1 : 2;
-    const int64_t ktg_need = n_g * Pp * esz;
-    if (ktg_need > p->ktg_cap) {
-        pfree(p->ktg);
-        const int64_t cap = ktg_need + ktg_need / 4;
-        unsigned char *buf = nullptr;
-        P_HIP(c, palloc(&buf, cap));
-        p->ktg = buf;
-        p->ktg_cap = cap;
-    }
     if (n_slots > p->slot_cap) {
         pfree(p->slot_ps);
         pfree(p->slot_pm);
@@ -362,10 +373,10 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
     return EMC_OK;
 }
 
-emc_status proposal_targets(emc_ctx *c, bool measure, double *captured)
+emc_status proposal_targets(emc_ctx *c, bool measure, bool logits, double *captured)
 {
     Proposal *p = c->prop;
-    P_HIP(c, mlp_cell_targets(c, p->Tt, c->stream));
+    P_HIP(c, mlp_cell_targets(c, p->Tt, logits && p->o.recall > 0 ? p->Mx : nullptr, c->stream));
     p->targets_valid = true;
     if (captured) *captured = -1.0;
     if (measure && p->pred_valid) {
@@ -381,21 +392,20 @@ emc_status proposal_targets(emc_ctx *c, bool measure, double *captured)
     return EMC_OK;
 }
 
-emc_status proposal_train(emc_ctx *c, int epochs, bool synth_only, double *loss)
+emc_status proposal_train(emc_ctx *c, int epochs, bool use_patterns, double syn_fraction, double *loss)
 {
     Proposal *p = c->prop;
     hipStream_t st = c->stream;
-    if (!synth_only && !p->targets_valid) return pfail(c, EMC_E_STATE, "proposal_train: no responsibility targets yet");
+    if (use_patterns && !p->targets_valid) return pfail(c, EMC_E_STATE, "proposal_train: no responsibility targets yet");
     const int I = c->I, B = p->o.batch;
-    // a refresh pass draws refresh_fraction of the synthetic patterns
-    const int n_syn = synth_only ? std::max(1, (int)std::lround(p->n_syn * (double)p->o.refresh_fraction)) * (p->n_syn > 0)
-                                 : p->n_syn;
-    if (synth_only && n_syn == 0) {
+    // each pass draws syn_fraction of the synthetic patterns afresh
+    const int n_syn = p->n_syn > 0 ? std::max(1, (int)std::lround(p->n_syn * syn_fraction)) : 0;
+    if (!use_patterns && n_syn == 0) {
         if (loss) *loss = -1.0;
         return EMC_OK;
     }
     // training items: patterns 0..I-1 (cell targets) then synthetic patterns I..I+n_syn-1
-    const int64_t N = synth_only ? n_syn : (int64_t)I + n_syn, first = synth_only ? I : 0;
+    const int64_t N = use_patterns ? (int64_t)I + n_syn : n_syn, first = use_patterns ? 0 : I;
     EmcRng rng(p->rng_state);
     std::vector<int32_t> idx(N + B), srot(std::max(n_syn, 1));
     double last = -1.0;
@@ -447,6 +457,8 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->batch = 256;
     o->synth_patterns = -1;
     o->refresh_epochs = 1;
+    o->recall = 4;
+    o->train_patterns = 1;
     o->refresh_fraction = 0.25f;
     o->min_mass = 0.99f;
     o->lr = 1e-3f;
@@ -571,6 +583,7 @@ static emc_status enable_impl(emc_ctx *c, const float *coarse, int32_t nc, const
     P_HIP(c, palloc(&p->d2, (int64_t)B * H));
     P_HIP(c, palloc(&p->loss_part, B));
     P_HIP(c, palloc(&p->Tt, (int64_t)Cp * Ip));
+    if (o.recall > 0) P_HIP(c, palloc(&p->Mx, (int64_t)Cp * Ip));
     P_HIP(c, palloc(&p->Xb, (int64_t)B * D));
     P_HIP(c, palloc(&p->tb, (int64_t)B * Cp));
     P_HIP(c, palloc(&p->bidx, (int64_t)I + p->n_syn + B));
@@ -578,8 +591,14 @@ static emc_status enable_impl(emc_ctx *c, const float *coarse, int32_t nc, const
     P_HIP(c, palloc(&p->Xs, (int64_t)syn_pad * D));
     P_HIP(c, palloc(&p->syn_rot, syn_pad));
     P_HIP(c, palloc(&p->topc, (int64_t)Ip * o.top_k));
+    P_HIP(c, hipHostMalloc((void **)&p->topc_h, sizeof(int32_t) * (size_t)I * o.top_k, hipHostMallocDefault));
+    P_HIP(c, hipHostMalloc((void **)&p->pmass_h, sizeof(float) * (size_t)I, hipHostMallocDefault));
+    P_HIP(c, hipHostMalloc((void **)&p->recall_h, sizeof(int32_t) * (size_t)I * std::max(o.recall, 1), hipHostMallocDefault));
+    P_HIP(c, hipEventCreateWithFlags(&p->ev_pred, hipEventDisableTiming));
     P_HIP(c, palloc(&p->pmass, Ip));
     P_HIP(c, palloc(&p->cap, Ip));
+    P_HIP(c, palloc(&p->recall, (int64_t)Ip * std::max(o.recall, 1)));
+    P_HIP(c, hipMemset(p->recall, 0xff, sizeof(int32_t) * (size_t)Ip * std::max(o.recall, 1)));  // -1: none yet
     P_HIP(c, palloc(&p->boff, nc));
     P_HIP(c, hipHostMalloc((void **)&p->boff_h, sizeof(int64_t) * nc, hipHostMallocDefault));
     P_HIP(c, hipStreamSynchronize(st));
@@ -605,7 +624,8 @@ extern "C" emc_status emc_proposal_enable(emc_ctx *c, const float *coarse, int32
     if (o.top_k < 1 || o.top_k > 64 || o.top_k > n_coarse || o.hidden < 64 || o.hidden % 64 || o.hidden > 4096 ||
         o.batch < 64 || o.batch % 64 || o.batch > 65536 || o.epochs < 0 || o.warmup_full < 1 || o.full_every < 1 ||
         !(o.lr > 0.0f) || !(o.min_mass >= 0.0f) || !(o.max_fallback >= 0.0f) || o.synth_patterns < -1 ||
-        o.refresh_epochs < 0 || o.refresh_epochs > 1000 || !(o.refresh_fraction > 0.0f && o.refresh_fraction <= 1.0f) ||
+        o.refresh_epochs < 0 || o.refresh_epochs > 1000 || o.recall < 0 || o.recall > kMaxRecall ||
+        (o.train_patterns != 0 && o.train_patterns != 1) || !(o.refresh_fraction > 0.0f && o.refresh_fraction <= 1.0f) ||
         o.synth_patterns > (1 << 22))
         return pfail(c, EMC_E_INVALID, "proposal_enable: bad option value");
     P_HIP(c, hipSetDevice(c->opt.device));
@@ -626,8 +646,8 @@ extern "C" emc_status emc_proposal_predict(emc_ctx *c, int32_t *cells, float *ma
     P_HIP(c, hipSetDevice(c->opt.device));
     P_TRY(proposal_predict(c));
     const Proposal *p = c->prop;
-    if (cells) std::memcpy(cells, p->topc_h.data(), sizeof(int32_t) * p->topc_h.size());
-    if (mass) std::memcpy(mass, p->pmass_h.data(), sizeof(float) * c->I);
+    if (cells) std::memcpy(cells, p->topc_h, sizeof(int32_t) * (size_t)c->I * p->o.top_k);
+    if (mass) std::memcpy(mass, p->pmass_h, sizeof(float) * c->I);
     return EMC_OK;
 }
 
@@ -641,13 +661,26 @@ extern "C" emc_status emc_proposal_captured(emc_ctx *c, double *cap)
     return EMC_OK;
 }
 
+extern "C" emc_status emc_proposal_recall(emc_ctx *c, int32_t *cells)
+{
+    P_TRY(need_prop(c, "proposal_recall"));
+    if (!cells) return pfail(c, EMC_E_INVALID, "proposal_recall: null argument");
+    const Proposal *p = c->prop;
+    if (p->o.recall == 0) return EMC_OK;
+    P_HIP(c, hipSetDevice(c->opt.device));
+    P_HIP(c, hipMemcpyAsync(cells, p->recall, sizeof(int32_t) * (size_t)c->I * p->o.recall, hipMemcpyDeviceToHost,
+                            c->stream));
+    P_HIP(c, hipStreamSynchronize(c->stream));
+    return EMC_OK;
+}
+
 extern "C" emc_status emc_proposal_cell_targets(emc_ctx *c, float *targets)
 {
     P_TRY(need_prop(c, "proposal_cell_targets"));
     if (!targets) return pfail(c, EMC_E_INVALID, "proposal_cell_targets: null argument");
     if (!c->has_resp) return pfail(c, EMC_E_STATE, "proposal_cell_targets: needs full-search responsibilities");
     P_HIP(c, hipSetDevice(c->opt.device));
-    P_TRY(proposal_targets(c, false, nullptr));
+    P_TRY(proposal_targets(c, false, false, nullptr));
     const Proposal *p = c->prop;
     std::vector<float> Tt((size_t)p->C * c->Ip);
     P_HIP(c, hipMemcpyAsync(Tt.data(), p->Tt, sizeof(float) * Tt.size(), hipMemcpyDeviceToHost, c->stream));
@@ -663,8 +696,8 @@ extern "C" emc_status emc_proposal_train(emc_ctx *c, int32_t epochs, double *los
     if (epochs < 0) return pfail(c, EMC_E_INVALID, "proposal_train: epochs must be >= 0");
     if (!c->has_resp) return pfail(c, EMC_E_STATE, "proposal_train: needs full-search responsibilities");
     P_HIP(c, hipSetDevice(c->opt.device));
-    P_TRY(proposal_targets(c, false, nullptr));
-    return proposal_train(c, epochs, false, loss);
+    P_TRY(proposal_targets(c, false, false, nullptr));
+    return proposal_train(c, epochs, true, 1.0, loss);
 }
 
 extern "C" emc_status emc_proposal_features(emc_ctx *c, float *raw)

@@ -250,7 +250,16 @@ typedef struct emc_proposal_options {
     int32_t synth_patterns; /* model-sliced training patterns per pass; -1 = n_patterns / 4 */
     int32_t refresh_epochs; /* passes over fresh model-sliced patterns after each proposal
                                iteration's compress, so the network follows the model (1) */
-    float min_mass;         /* per-pattern predicted mass below which it is searched fully (0.99) */
+    int32_t recall;         /* cells searched beside the network's top_k: the pattern's cells of
+                               largest max logit in its most recent search (all cells after a
+                               full search, the searched ones after a proposal iteration), so
+                               a runner-up orientation that overtakes the best is still found;
+                               0..8 (4) */
+    int32_t train_patterns; /* 1: training after a full search uses the patterns' cell targets
+                               besides the model-sliced patterns; 0: model-sliced patterns only,
+                               so the network learns the current model's orientation map rather
+                               than the previous assignments (1) */
+    float min_mass;        /* per-pattern predicted mass below which it is searched fully (0.99) */
     float lr;               /* Adam step size (1e-3; beta 0.9 / 0.999, eps 1e-8) */
     float max_fallback;     /* fraction of fully searched patterns above which the iteration
                                runs a full search instead (0.1) */
@@ -268,12 +277,16 @@ emc_status emc_coarse_partition(const float *fine, int32_t n_fine, const float *
 emc_status emc_proposal_enable(emc_ctx *ctx, const float *coarse, int32_t n_coarse,
                                const emc_proposal_options *opt);
 emc_status emc_proposal_disable(emc_ctx *ctx);
-/* Network prediction for every pattern: cells [I][top_k] (ascending), mass [I].  Either may be
- * NULL. */
+/* Network prediction for every pattern: cells [I][top_k] (ascending), mass [I] = the predicted
+ * probability of every cell a proposal iteration would search (the top_k and the recalled
+ * ones).  Either may be NULL. */
 emc_status emc_proposal_predict(emc_ctx *ctx, int32_t *cells, float *mass);
 /* Per-pattern captured mass of the last full-search iteration that measured it (the
- * emc_iter_stats.captured_mass average): cap [I]. */
+ * emc_iter_stats.captured_mass average; the cells counted are the top_k predicted ones and the
+ * recalled ones): cap [I]. */
 emc_status emc_proposal_captured(emc_ctx *ctx, double *cap);
+/* Recalled cells of every pattern, most massive first, -1 for none: cells [I][recall]. */
+emc_status emc_proposal_recall(emc_ctx *ctx, int32_t *cells);
 /* Cell sums of the current full responsibilities: targets [I][n_coarse]. */
 emc_status emc_proposal_cell_targets(emc_ctx *ctx, float *targets);
 /* Train on the current full responsibilities (cell targets) plus synthetic patterns for

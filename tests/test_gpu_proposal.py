@@ -174,6 +174,62 @@ def test_trained_network_captures_mass_and_beats_untrained(d):
             assert rel < 5e-2
 
 
+def check_top_logit_cells(e, d, rec, R):
+    """rec [I][R] holds, for each pattern, R distinct cells of largest max logit (the logits of
+    the last full search; a tolerance of a few float roundings at the R-th value)."""
+    cell = emc_amd.coarse_partition(d["q"], d["cq"])
+    Lg = e.get_logl().astype(np.float64)
+    C = d["cq"].shape[0]
+    mx = np.full((C, Lg.shape[1]), -np.inf)
+    for c in range(C):
+        if (cell == c).any():
+            mx[c] = Lg[cell == c].max(0)
+    thr = -np.sort(-mx, axis=0)[R - 1]
+    assert (rec >= 0).all()
+    assert all(len(set(r)) == R for r in rec.tolist())
+    got = np.take_along_axis(mx, rec.T.astype(np.int64), 0).min(0)
+    assert (got >= thr - 1e-5 * np.maximum(1.0, np.abs(thr))).all()
+
+
+def test_recall_follows_the_search(d):
+    """Recalled cells: after a full search R cells of largest max logit; after a proposal
+    iteration a subset of the cells it searched; the captured mass a full search measures counts
+    the predicted and the recalled cells once each."""
+    R, k = 3, 4
+    log(f"[recall] top_k={k} recall={R} of 60 cells, 1,000 patterns")
+    with engine(d) as e:
+        e.enable_proposal(d["cq"], top_k=k, recall=R, warmup_full=1, full_every=3, epochs=2, hidden=64, batch=64,
+                          min_mass=0.0, max_fallback=1.0, seed=4)
+        assert (e.proposal_recall() == -1).all()
+        assert e.iterate(0)["search"] == "full"
+        rec = e.proposal_recall()
+        check_top_logit_cells(e, d, rec, R)
+        for it in (1, 2):  # proposal iterations (full_every=3 after one warm-up iteration)
+            cells, _ = e.proposal_predict()
+            s = e.iterate(it)
+            assert s["search"] == "proposal"
+            new = e.proposal_recall()
+            for i in range(new.shape[0]):
+                searched = set(cells[i].tolist()) | set(rec[i][rec[i] >= 0].tolist())
+                got = new[i][new[i] >= 0].tolist()
+                assert len(got) >= 1 and set(got) <= searched, (it, i, got, searched)
+            log(f"  it {it}: work {s['work_fraction']:.3f}, recalled per pattern {float((new >= 0).sum(1).mean()):.2f}")
+            rec = new
+        cells, _ = e.proposal_predict()
+        s = e.iterate(3)
+        assert s["search"] == "full"
+        t = e.proposal_cell_targets()
+        cap = e.proposal_captured()
+        want = np.zeros(t.shape[0])
+        for i in range(t.shape[0]):
+            cs = set(cells[i].tolist()) | set(rec[i][rec[i] >= 0].tolist())
+            want[i] = sum(float(t[i, c]) for c in cs)
+        log(f"  it 3: captured {s['captured_mass']:.4f} (top-{k} alone {float(np.take_along_axis(t, cells, 1).sum(1).mean()):.4f})")
+        assert np.allclose(cap, want, atol=1e-5)
+        assert abs(s["captured_mass"] - want.mean()) < 1e-6
+        check_top_logit_cells(e, d, e.proposal_recall(), R)
+
+
 def test_proposal_rejects_bad_use(d):
     log("[proposal errors]")
     with emc_amd.Engine(d["L"], d["q"].shape[0], 64, precision="fp32") as e:
@@ -185,6 +241,8 @@ def test_proposal_rejects_bad_use(d):
             e.enable_proposal(d["cq"], top_k=61)
         with pytest.raises(emc_amd.EmcError, match="INVALID"):
             e.enable_proposal(d["cq"], hidden=100)
+        with pytest.raises(emc_amd.EmcError, match="INVALID"):
+            e.enable_proposal(d["cq"], recall=9)
         with pytest.raises(emc_amd.EmcError, match="STATE"):
             e.proposal_predict()
         e.enable_proposal(d["cq"], hidden=64, batch=64)
