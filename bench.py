@@ -9,7 +9,7 @@ beside it (`fp32`), and the FP64 CPU oracle on a rotation subset is the `cpu_bas
 and 50,000 patterns (64^3 model, 2,000 photons): the same iterations from the same start.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload NAME] [--no-cpu] [--no-fp32]
-                    [--no-proposal] [--proposal-iters 13] [--accuracy] [--variants]
+                    [--no-proposal] [--proposal-iters 23] [--accuracy] [--variants]
 """
 from __future__ import annotations
 
@@ -214,6 +214,12 @@ def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: b
                         ok = rr >= 0
                         np.put_along_axis(sel, np.where(ok, rr, pred_before[:, :1]), True, 1)
                     union = (tg * sel).sum(1)
+                    row["capture_network_only"] = float(np.take_along_axis(tg, pred_before.astype(np.int64), 1).sum(1).mean())
+                    if rec_before.shape[1]:
+                        rsel = np.zeros_like(sel)
+                        ii, qq = np.nonzero(ok)
+                        rsel[ii, rr[ii, qq]] = True
+                        row["capture_recall_only"] = float((tg * rsel).sum(1).mean())
                     row["capture_by_threshold"] = {
                         f"{t:g}": [float((pm_before < t).mean()), float(np.where(pm_before < t, 1.0, union).mean())]
                         for t in (0.0, 0.5, 0.9, 0.95, 0.98, 0.99, 0.995)}
@@ -261,13 +267,17 @@ def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: b
     prop_ms = [r["ms"] for r in rows if r["search"] == "proposal"]
     prop_full_ms = [r["ms"] for r in rows if r["search"] == "full" and r["it"] >= 3]
     caps = [r["captured_mass"] for r in rows if r["captured_mass"] >= 0]
+    # the capture a full search measures right after proposal iterations: what they really missed
+    caps_after = [r["captured_mass"] for k, r in enumerate(rows)
+                  if r["captured_mass"] >= 0 and k > 0 and rows[k - 1]["search"] == "proposal"]
     ref_ms = float(np.mean(full_ms[1:])) if len(full_ms) > 1 else float(full_ms[0])
     out = {"workload": name, "fine_rotations": S, "coarse_cells": int(cq.shape[0]), "patterns": I,
-           "iterations": iters, "full_search_ms": ref_ms,
+           "iterations": iters, "options": options or {}, "full_search_ms": ref_ms,
            "proposal_iteration_ms": float(np.mean(prop_ms)) if prop_ms else None,
            "speedup_proposal_iteration": ref_ms / float(np.mean(prop_ms)) if prop_ms else None,
            "full_with_training_ms": float(np.mean(prop_full_ms)) if prop_full_ms else None,
-           "captured_mass": caps, "final_model_rel_l2_vs_full": rows[-1]["model_rel_l2_vs_full"],
+           "captured_mass": caps, "captured_mass_after_proposals": caps_after,
+           "final_model_rel_l2_vs_full": rows[-1]["model_rel_l2_vs_full"],
            "per_iteration": rows}
     if prop_ms and prop_full_ms:
         # steady state over one period of the policy: (full_every - 1) proposal + 1 full iteration
@@ -292,7 +302,7 @@ def main():
     ap.add_argument("--no-proposal", action="store_true",
                     help="skip the learned proposal at 204,960 rotations against the full search")
     ap.add_argument("--proposal-only", action="store_true", help="only the learned-proposal comparison")
-    ap.add_argument("--proposal-iters", type=int, default=13)
+    ap.add_argument("--proposal-iters", type=int, default=23)
     ap.add_argument("--proposal-options", default="{}", help="JSON object of emc_proposal_options overrides")
     ap.add_argument("--diagnose", action="store_true", help="proposal: where the missed responsibility mass went")
     args = ap.parse_args()

@@ -202,7 +202,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
-    dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_units);
+    dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);

@@ -169,6 +169,8 @@ struct emc_ctx {
     // binned insertion: (s, j) pair indices sorted by (chunk, model brick); work units
     uint32_t *bin_vals = nullptr;                      // [S * Pa] pair index s * Pa + j, (s, j) order per unit
     uint32_t *bin_perm = nullptr;                      // [S * Pa] positions in bin_vals, base-voxel order per unit
+    uint16_t *bin_vox = nullptr;                       // [S * Pa] base voxel in the unit's halo brick (perm order)
+    float *bin_frac = nullptr;                         // [S * Pa][3] trilinear fractions (perm order)
     int32_t *bin_units = nullptr;                      // [n_units][3] = key, begin, end
     std::vector<int> bin_chunk_units;                  // unit range of each rotation chunk
     bool bins_valid = false;

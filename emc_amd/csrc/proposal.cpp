@@ -457,7 +457,7 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->batch = 256;
     o->synth_patterns = -1;
     o->refresh_epochs = 1;
-    o->recall = 4;
+    o->recall = 8;
     o->train_patterns = 1;
     o->refresh_fraction = 0.25f;
     o->min_mass = 0.99f;

@@ -254,7 +254,7 @@ typedef struct emc_proposal_options {
                                largest max logit in its most recent search (all cells after a
                                full search, the searched ones after a proposal iteration), so
                                a runner-up orientation that overtakes the best is still found;
-                               0..8 (4) */
+                               0..8 (8) */
     int32_t train_patterns; /* 1: training after a full search uses the patterns' cell targets
                                besides the model-sliced patterns; 0: model-sliced patterns only,
                                so the network learns the current model's orientation map rather

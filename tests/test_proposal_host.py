@@ -53,6 +53,6 @@ def test_proposal_defaults():
     o = emc_amd.proposal_options()
     assert (o.top_k, o.warmup_full, o.full_every, o.hidden, o.batch) == (8, 3, 5, 512, 256)
     assert abs(o.min_mass - 0.99) < 1e-6 and o.synth_patterns == -1
-    assert (o.recall, o.refresh_epochs) == (4, 1)
+    assert (o.recall, o.refresh_epochs, o.train_patterns) == (8, 1, 1)
     with pytest.raises(KeyError):
         emc_amd.proposal_options(nonsense=1)
