@@ -27,13 +27,17 @@ WORKLOADS = {
 }
 
 
-def make(name: str, I: int | None = None, rot_n: int | None = None, with_rotations: bool = True) -> dict:
-    """Arrays of a workload; I / rot_n override the pattern count / rotation sampling."""
+def make(name: str, I: int | None = None, rot_n: int | None = None, with_rotations: bool = True,
+         photons: float | None = None) -> dict:
+    """Arrays of a workload; I / rot_n / photons override the pattern count / rotation sampling /
+    mean photons per pattern."""
     cfg = dict(WORKLOADS[name])
     if I is not None:
         cfg["I"] = I
     if rot_n is not None:
         cfg["rot_n"] = rot_n
+    if photons is not None:
+        cfg["photons"] = photons
     L = cfg["L"]
     seed = cfg["seed"]
     model_true = synth_model(L, cfg["n_atoms"], cfg["radius"], 1000 + seed)

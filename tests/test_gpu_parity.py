@@ -187,6 +187,37 @@ def test_full_iterations_match_oracle(orc, small, prec):
             m = ref["model"].astype(np.float32)
 
 
+def test_sparse_photon_iterations_match_oracle(orc):
+    """The sparse-count regime of the 128^3 / 300-photon workload (about 0.023 photons per active
+    pixel, counts mostly 0 or 1) at the small shape: 18 photons over 780 pixels; byte counts on the
+    MFMA path, two iterations against the oracle."""
+    d = wl.make("L32_S3240_I1000", photons=18.0)
+    K = d["K"]
+    zeros = float((K[:, d["mask"] != 0] == 0).mean())
+    log(f"[sparse] max count {K.max()}, zero fraction {zeros:.3f}")
+    assert zeros > 0.95 and K.max() <= 16
+    L = d["L"]
+    m = wl.normalized(d["model0"]).astype(np.float32)
+    K64 = K.astype(np.float64)
+    w64 = d["w"].astype(np.float64)
+    with emc_amd.Engine(L, d["q"].shape[0], K.shape[0], precision="bf16x2", blur_sigma=0.5) as e:
+        e.setup(d["q"], d["w"], d["coords"], d["mask"], K, m)
+        for it in range(2):
+            e.set_model(m)
+            st = e.iterate(it)
+            M, _ = e.get_model()
+            P = e.get_resp()
+            ref = orc.iterate(m.astype(np.float64), L, d["q"], w64, K64, d["mask"], d["coords"], "trilinear",
+                              "true_poisson", blur_sigma=0.5, want_P=True)
+            mt = metrics(None, None, P, ref["P"], M, ref["model"])
+            rel_tot = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
+            log(f"   it {it}: {mt} total logL rel {rel_tot:.2e}")
+            assert rel_tot < 1e-5
+            assert mt["resp_l1_max"] < 5e-3
+            assert mt["model_rel_l2"] < 1e-4
+            m = ref["model"].astype(np.float32)
+
+
 def test_chunked_rotations_equal_unchunked(small):
     log("[chunk]")
     d = small
