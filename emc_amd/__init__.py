@@ -11,7 +11,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (DIFF, INSERT, INTERP, PRECISION, SEARCH, STAGES, Dims, EmcError, IterStats, Options,
+from ._lib import (DIFF, INSERT, INTERP, MSTEP, PRECISION, SEARCH, STAGES, Dims, EmcError, IterStats, Options,
                    ProposalOptions, check, lib)
 
 __all__ = ["Engine", "EmcError", "rotations", "rotation_count", "coords_flat", "coords_ewald", "mask_annulus",
@@ -101,6 +101,8 @@ def default_options(**kw) -> Options:
             v = PRECISION[v] if isinstance(v, str) else v
         elif k == "insert":
             v = INSERT[v] if isinstance(v, str) else v
+        elif k == "mstep":
+            v = MSTEP[v] if isinstance(v, str) else v
         if not hasattr(o, k):
             raise KeyError(k)
         setattr(o, k, v)

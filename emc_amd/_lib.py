@@ -20,6 +20,7 @@ DIFF = {"absolute": 0, "poisson": 1, "relative": 2, "true_poisson": 3}
 INTERP = {"nearest": 0, "trilinear": 1}
 PRECISION = {"fp32": 0, "bf16x2": 1, "bf16x3": 2}
 INSERT = {"atomic": 0, "fused": 1, "binned": 2}
+MSTEP = {"auto": 0, "dense": 1, "sparse": 2}
 N_STAGES = 10
 STAGES = ["slice", "logl", "softmax", "mstep", "compress", "total", "logl_gemm", "mstep_gemm", "propose", "train"]
 SEARCH = {0: "full", 1: "proposal"}
@@ -35,7 +36,7 @@ class Options(C.Structure):
                 ("sigma_start", C.c_double), ("sigma_final", C.c_double), ("sigma_half_life", C.c_int32),
                 ("device", C.c_int32), ("blur_sigma", C.c_double), ("fluence", C.c_double),
                 ("timing", C.c_int32), ("mfma_tile", C.c_int32), ("count_bytes", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("mstep", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class IterStats(C.Structure):

@@ -223,6 +223,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad dimensions");
     if (opt.diff < 0 || opt.diff > 3 || opt.interp < 0 || opt.interp > 1 || opt.precision < 0 || opt.precision > 2 ||
         opt.insert < 0 || opt.insert > 2 || (opt.count_bytes != 0 && opt.count_bytes != 2) || opt.mfma_tile < 0 ||
+        opt.mstep < 0 || opt.mstep > 2 ||
         opt.mfma_tile > 1)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
     if (opt.precision != EMC_PREC_FP32 && opt.diff != EMC_DIFF_TRUE_POISSON)
@@ -714,10 +715,13 @@ extern "C" emc_status emc_set_resp(emc_ctx *c, const float *resp)
         c->has_logl = false;
     } else {
         std::vector<bf16_t> h((size_t)S * I), l((size_t)S * I);
+        double nnz = 0.0;  // read by the M-step's choice of form, as after an E-step
         for (size_t k = 0; k < h.size(); ++k) {
             h[k] = f2bf(resp[k]);
             l[k] = f2bf(resp[k] - bf2f(h[k]));
+            nnz += (h[k] | l[k]) != 0;
         }
+        HIP_TRY(c, hipMemcpy(c->scal + SC_NNZ, &nnz, sizeof(double), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy2D(c->Ph, sizeof(bf16_t) * c->Ip, h.data(), sizeof(bf16_t) * I, sizeof(bf16_t) * I, S,
                                hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy2D(c->Pl, sizeof(bf16_t) * c->Ip, l.data(), sizeof(bf16_t) * I, sizeof(bf16_t) * I, S,
@@ -735,12 +739,23 @@ static emc_status mstep_enqueue(emc_ctx *c)
     HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
     HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
     const bool fused = c->mfma && c->opt.insert == EMC_INSERT_FUSED;
+    // MFMA M-step form: both kernels are enqueued in AUTO mode and the one not chosen by the
+    // E-step's non-zero count (scal[SC_NNZ]) returns at once, so the host never waits for it
+    const int mode = fused || c->Pp > kSparseMaxPp ? EMC_MSTEP_DENSE : c->opt.mstep;
+    const double thr = 0.02 * (double)c->S * (double)c->I;
+    const double *gate = c->scal + SC_NNZ;
     for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
         const int cs = std::min(c->chunk, c->S - s0);
         const int k = s0 / c->chunk;
         gev_rec(c, 1, k, 0);
-        if (c->mfma) HIP_TRY(c, launch_mstep_mfma(c, s0, cs, fused, st));
-        else HIP_TRY(c, launch_mstep_f32(c, s0, cs, st));
+        if (c->mfma) {
+            if (mode != EMC_MSTEP_DENSE)
+                HIP_TRY(c, launch_mstep_sparse(c, s0, cs, mode == EMC_MSTEP_AUTO ? gate : nullptr, thr, st));
+            if (mode != EMC_MSTEP_SPARSE)
+                HIP_TRY(c, launch_mstep_mfma(c, s0, cs, fused, mode == EMC_MSTEP_AUTO ? gate : nullptr, thr, st));
+        } else {
+            HIP_TRY(c, launch_mstep_f32(c, s0, cs, st));
+        }
         gev_rec(c, 1, k, 1);
         if (c->opt.insert == EMC_INSERT_BINNED) HIP_TRY(c, launch_insert_binned(c, s0, cs, st));
         else if (!fused) HIP_TRY(c, launch_insert(c, s0, cs, st));

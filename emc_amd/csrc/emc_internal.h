@@ -33,6 +33,7 @@ enum {
     SC_SUMW = 9,      // sum over rotations of the slice sums
     SC_B0 = 10,       // global logit shift removed from the stored logits (true Poisson)
     SC_LOSS = 11,     // proposal network: summed cross-entropy of a training pass
+    SC_NNZ = 12,      // non-zero responsibilities of the last dense E-step (BF16 hi or lo non-zero)
     SC_N = 16
 };
 
@@ -204,7 +205,14 @@ hipError_t launch_logl_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
 hipError_t launch_mstep_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
 // MFMA contractions.
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st);
-hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipStream_t st);
+// The dense M-step returns at once when gate is non-null and *gate <= gate_thr.
+hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const double *gate, double gate_thr,
+                             hipStream_t st);
+// Sparse M-step: U rows [s0, s0+cs) summed over the non-zero responsibilities of each rotation
+// (same U as the dense contraction up to summation order); returns at once when gate is non-null
+// and *gate > gate_thr.  Needs Pp <= kSparseMaxPp.
+constexpr int kSparseMaxPp = 16384;
+hipError_t launch_mstep_sparse(const emc_ctx *c, int s0, int cs, const double *gate, double gate_thr, hipStream_t st);
 
 // Responsibilities.  When logl_fuses_softmax(c), the logL contraction's epilogue writes the
 // per-band softmax partials (one band per 256 rotation rows) and launch_softmax skips its pass.

@@ -45,6 +45,10 @@ struct EpiArgs {
     float *pm;
     double *ps;
     int64_t band0, ldp;
+    // device-side choice between this kernel and the sparse M-step: the whole grid returns at
+    // once when *gate <= gate_thr (the responsibilities' non-zero count is small)
+    const double *gate;
+    double gate_thr;
 };
 
 __device__ __forceinline__ void scatter_one(const EpiArgs &e, const float *m, float Ts, int j, float u)
@@ -83,6 +87,7 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const bf16_t *__restrict__ A0
                                                    const bf16_t *__restrict__ B0, const bf16_t *__restrict__ B1,
                                                    int64_t ldb, int Kd, EpiArgs e)
 {
+    if (e.gate && *e.gate <= e.gate_thr) return;
     constexpr int kTileBytes = (NA + NB) * BM * LDK * 2;
     static_assert(kTileBytes >= 64 * (BN + 4) * 4, "LDS too small for the epilogue staging");
     __shared__ __attribute__((aligned(16))) unsigned char smem[kTileBytes];
@@ -255,6 +260,7 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
                                                     const void *__restrict__ Bv, int64_t ldb, int Kd, int ntn,
                                                     int n_base, EpiArgs e)
 {
+    if (e.gate && *e.gate <= e.gate_thr) return;
     constexpr int TN = BN2 / 64;                       // 32-wide tiles per wave along N
     constexpr int BCH = BN2 * (BK2 / 8) / 512;         // 16-byte B chunks per thread per stage
     constexpr int STAGE = (NA * BM2 + BN2) * LDK2;     // BF16 elements per stage
@@ -488,9 +494,12 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
     return run<2, 1, EPI_LOGL>(grid, st, c->Lh, A1, nullptr, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);
 }
 
-hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, hipStream_t st)
+hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const double *gate, double gate_thr,
+                             hipStream_t st)
 {
     EpiArgs e{};
+    e.gate = fused ? nullptr : gate;
+    e.gate_thr = gate_thr;
     e.C = c->U;
     e.ldc = c->Pp;
     e.m_valid = fused ? cs : c->chunkp;

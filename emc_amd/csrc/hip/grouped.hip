@@ -19,6 +19,7 @@
 #include <algorithm>
 
 #include "../emc_internal.h"
+#include "sparse_row.h"
 
 namespace emc {
 namespace {
@@ -69,8 +70,10 @@ template <int MODE, int NA, bool BU8>
 __global__ __launch_bounds__(256) void k_grouped(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
                                                  const bf16_t *__restrict__ A2, int64_t lda,
                                                  const void *__restrict__ Bv, int64_t ldb, int Pp, GTab g,
-                                                 const float *__restrict__ bias, float *__restrict__ out)
+                                                 const float *__restrict__ bias, float *__restrict__ out,
+                                                 const double *__restrict__ gate, double gate_thr)
 {
+    if (MODE == G_MSTEP && gate && *gate <= gate_thr) return;  // the sparse M-step runs instead
     constexpr int STAGE = (NA * TM + TN) * TLD;
     __shared__ __attribute__((aligned(16))) bf16_t smem[2 * STAGE];
     __shared__ float sbias[TM];
@@ -295,28 +298,58 @@ __global__ void k_slot_lse(const int32_t *__restrict__ slot_pat, const double *_
 // pattern selected get T = 0, which the insertion skips.
 __global__ __launch_bounds__(256) void k_gsm_apply(const float *__restrict__ Lb, GTab g, int S,
                                                    const double *__restrict__ slse, bf16_t *__restrict__ Ph,
-                                                   bf16_t *__restrict__ Pl, double *__restrict__ T)
+                                                   bf16_t *__restrict__ Pl, double *__restrict__ T,
+                                                   double *__restrict__ nzrow)
 {
     const int gi = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (gi >= S) return;
     const int rot = g.cell_rot[gi], c = g.cell_of[rot], r = gi - g.rbeg[c];
     const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
     if (ns == 0) {
-        if (lane == 0) T[rot] = 0.0;
+        if (lane == 0) {
+            T[rot] = 0.0;
+            nzrow[gi] = 0.0;
+        }
         return;
     }
     const int64_t row = g.boff[c] + (int64_t)r * nsp;
     double acc = 0.0;
+    int nz = 0;  // non-zero P of the row (for the choice of the M-step form)
     for (int s = lane; s < nsp; s += 64) {
         float p = 0.0f;
         if (s < ns) p = expf((float)((double)Lb[row + s] - slse[sb + s]));
-        const bf16_t h = f2bf(p);
+        const bf16_t h = f2bf(p), lo = f2bf(p - bf2f(h));
         Ph[row + s] = h;
-        Pl[row + s] = f2bf(p - bf2f(h));
+        Pl[row + s] = lo;
         acc += (double)p;
+        nz += (h | lo) != 0;
     }
     acc = wave_sum(acc);
-    if (lane == 0) T[rot] = acc;
+    nz = wave_sum(nz);
+    if (lane == 0) {
+        T[rot] = acc;
+        nzrow[gi] = (double)nz;
+    }
+}
+
+// Sparse M-step of a proposal iteration: the U row of each fine rotation of a selected cell from
+// the non-zero responsibilities of its row of the cell's P block (sparse_row.h; entry n of the row
+// is the pattern slot_pat[sbeg[c] + n]).  One workgroup per fine rotation, in cell order; rotations
+// of cells no pattern selected are left alone (their T is 0 and the insertion skips them).
+template <int KT>
+__global__ __launch_bounds__(256) void k_gmstep_sparse(const bf16_t *__restrict__ Ph, const bf16_t *__restrict__ Pl,
+                                                       GTab g, const void *__restrict__ Kv,
+                                                       const bf16_t *__restrict__ Klo, int Pp,
+                                                       const double *__restrict__ gate, double gate_thr,
+                                                       float *__restrict__ U)
+{
+    if (gate && *gate > gate_thr) return;  // the dense grouped contraction runs instead
+    const int gi = blockIdx.x;
+    const int rot = g.cell_rot[gi], c = g.cell_of[rot], r = gi - g.rbeg[c];
+    const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
+    if (ns == 0) return;
+    const int64_t row = g.boff[c] + (int64_t)r * nsp;
+    sparse_row<KT>(Ph + row, Pl + row, ns, g.slot_pat + sb, Kv, Klo, Pp, U + (size_t)rot * Pp);
 }
 
 // The kMaxRecall cells of largest score seen so far, sorted by score (descending; on equal
@@ -390,6 +423,9 @@ __global__ __launch_bounds__(256) void k_recall_slots(const float *__restrict__ 
     t.store(out + (size_t)i * R, R);
 }
 
+// Doubles at the start of c->part used by the grouped softmax's per-block lse sums.
+int64_t blk_pad(const emc_ctx *c) { return ((c->I + 255) / 256 + 16) / 16 * 16; }
+
 GTab tables(const emc_ctx *c, const int4 *tiles)
 {
     const Proposal *p = c->prop;
@@ -422,17 +458,17 @@ hipError_t launch_grouped_logl(const emc_ctx *c, hipStream_t st)
     if (c->K8) {
         if (c->la_terms == 3)
             hipLaunchKernelGGL((k_grouped<G_LOGL, 3, true>), grid, blk, 0, st, c->Lh, A1, c->Ll, ld, c->K8, ld, c->Pp, g,
-                               c->bias, c->R);
+                               c->bias, c->R, nullptr, 0.0);
         else
             hipLaunchKernelGGL((k_grouped<G_LOGL, 2, true>), grid, blk, 0, st, c->Lh, A1, nullptr, ld, c->K8, ld, c->Pp,
-                               g, c->bias, c->R);
+                               g, c->bias, c->R, nullptr, 0.0);
     } else {
         if (c->la_terms == 3)
             hipLaunchKernelGGL((k_grouped<G_LOGL, 3, false>), grid, blk, 0, st, c->Lh, A1, c->Ll, ld, c->K16, ld, c->Pp,
-                               g, c->bias, c->R);
+                               g, c->bias, c->R, nullptr, 0.0);
         else
             hipLaunchKernelGGL((k_grouped<G_LOGL, 2, false>), grid, blk, 0, st, c->Lh, A1, nullptr, ld, c->K16, ld,
-                               c->Pp, g, c->bias, c->R);
+                               c->Pp, g, c->bias, c->R, nullptr, 0.0);
     }
     return hipGetLastError();
 }
@@ -450,9 +486,10 @@ hipError_t launch_grouped_softmax(const emc_ctx *c, hipStream_t st)
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_slot_lse, dim3((unsigned)((p->n_slots + 255) / 256)), dim3(256), 0, st, g.slot_pat, c->lse,
                        p->n_slots, p->slot_lse);
+    double *nzrow = c->part + blk_pad(c);  // one non-zero count per fine rotation
     hipLaunchKernelGGL(k_gsm_apply, dim3((c->S + 3) / 4), dim3(256), 0, st, c->R, g, c->S, p->slot_lse, c->Ph, c->Pl,
-                       c->T);
-    return hipGetLastError();
+                       c->T, nzrow);
+    return launch_sum_f64(nzrow, c->S, c->scal + SC_NNZ, st);
 }
 
 hipError_t launch_recall_grouped(const emc_ctx *c, hipStream_t st)
@@ -480,14 +517,31 @@ hipError_t launch_grouped_mstep(const emc_ctx *c, hipStream_t st)
 {
     const Proposal *p = c->prop;
     if (p->n_tiles_m == 0) return hipSuccess;
+    // dense or sparse form, chosen on the device from the non-zero count of the grouped softmax
+    // (at most 2 % of the block elements non-zero: sparse), unless the options fix one
+    const int mode = c->Pp > kSparseMaxPp ? EMC_MSTEP_DENSE : c->opt.mstep;
+    const double *gate = mode == EMC_MSTEP_AUTO ? c->scal + SC_NNZ : nullptr;
+    const double thr = 0.02 * (double)p->n_blk;
+    if (mode != EMC_MSTEP_DENSE) {
+        const GTab gs = tables(c, nullptr);
+        if (c->K8)
+            hipLaunchKernelGGL(k_gmstep_sparse<0>, dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K8, nullptr, c->Pp,
+                               gate, thr, c->U);
+        else
+            hipLaunchKernelGGL(k_gmstep_sparse<1>, dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K16, nullptr,
+                               c->Pp, gate, thr, c->U);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (mode == EMC_MSTEP_SPARSE) return hipSuccess;
     const GTab g = tables(c, tile_list(p, p->off_tile_m));
     const dim3 grid(p->n_tiles_m), blk(256);
     if (c->K8)
         hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, true>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, c->K8, c->Pp, c->Pp,
-                           g, nullptr, c->U);
+                           g, nullptr, c->U, gate, thr);
     else
         hipLaunchKernelGGL((k_grouped<G_MSTEP, 2, false>), grid, blk, 0, st, c->Ph, c->Pl, nullptr, 0, c->K16, c->Pp, c->Pp,
-                           g, nullptr, c->U);
+                           g, nullptr, c->U, gate, thr);
     return hipGetLastError();
 }
 

@@ -7,7 +7,8 @@
 //      online (max, sum of exp) pair; exp is FP32, the sum FP64.  Reads are coalesced rows.
 //   2. final:   combine the bands per pattern in FP64 -> lse'_i; per-block partial sums of lse'.
 //   3. apply:   P = exp(R - lse'_i) written either in place (FP32 path) or as a BF16 hi+lo pair
-//      (MFMA path), plus per-band row sums of P (T_s partials), reduced in FP64.
+//      (MFMA path), plus per-band row sums of P (T_s partials), reduced in FP64, and (MFMA path)
+//      the count of non-zero P, which picks the dense or the sparse form of the M-step.
 // The reference's -1e10 floor on the max and on R - max is kept.
 #include <algorithm>
 
@@ -96,8 +97,10 @@ constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
 template <int OUT>
 __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t ldr, int S, int I,
                                                   const double *__restrict__ lse, bf16_t *__restrict__ Ph,
-                                                  bf16_t *__restrict__ Pl, double *__restrict__ Tpart)
+                                                  bf16_t *__restrict__ Pl, double *__restrict__ Tpart,
+                                                  double *__restrict__ nnz)
 {
+    int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
     __shared__ double red[4][kApplyRows];
     const int c0 = blockIdx.x * kApplyCols + threadIdx.x * 4;
     const int r0 = blockIdx.y * kApplyRows;
@@ -127,6 +130,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                 for (int k = 0; k < 4; ++k) {
                     h[k] = f2bf(p[k]);
                     lo[k] = f2bf(p[k] - bf2f(h[k]));
+                    nz += (h[k] | lo[k]) != 0;
                 }
                 const size_t o = (size_t)s * ldr + c0;
                 *reinterpret_cast<uint2 *>(Ph + o) =
@@ -143,12 +147,20 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         const double t = wave_sum(rowsum[rr]);
         if (lane == 0) red[wave][rr] = t;
     }
+    __shared__ int nzw[4];
+    if (OUT == 1) {
+        nz = wave_sum(nz);
+        if (lane == 0) nzw[wave] = nz;
+    }
     __syncthreads();
     if (threadIdx.x < kApplyRows) {
         const int s = r0 + threadIdx.x;
         const int k = threadIdx.x;
         if (s < S) Tpart[(size_t)blockIdx.x * S + s] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
     }
+    // per-workgroup non-zero count, summed by one deterministic pass afterwards
+    if (OUT == 1 && threadIdx.x == 0)
+        nnz[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = (double)(nzw[0] + nzw[1] + nzw[2] + nzw[3]);
 }
 
 // T_s = sum over column blocks of the partial row sums (deterministic order).
@@ -164,7 +176,8 @@ __global__ void k_T_final(const double *__restrict__ Tpart, int ncb, int S, int 
 
 }  // namespace
 
-// Partials layout inside c->part:  pm (float, nb*I) | ps (double, nb*I) | blk (double) | Tpart.
+// Partials layout inside c->part:  pm (float, nb*I) | ps (double, nb*I) | blk (double) | Tpart |
+// non-zero counts (one per apply workgroup).
 // nb = the partial pass's band count, or (MFMA logL with fused partials) one band per 256 rows.
 int softmax_bands(int S, int I)
 {
@@ -176,7 +189,7 @@ int64_t softmax_part_len(int S, int I, int fused_bands)
     const int64_t nb = std::max(softmax_bands(S, I), fused_bands);
     const int ncb = (I + kApplyCols - 1) / kApplyCols;
     return nb * I /*pm as floats packed in doubles: over-allocate*/ + nb * I + (I + 255) / 256 + 1 +
-           (int64_t)ncb * S + 16;
+           (int64_t)ncb * S + (int64_t)ncb * ((S + kApplyRows - 1) / kApplyRows) + 16;
 }
 
 hipError_t launch_sum_f64(const double *x, int n, double *out, hipStream_t st)
@@ -203,13 +216,15 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
     hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, blk, ncol, c->scal + SC_TOTAL);
     const int ncb = (I + kApplyCols - 1) / kApplyCols;
     const dim3 ga(ncb, (S + kApplyRows - 1) / kApplyRows);
+    double *nzpart = Tpart + (int64_t)ncb * S;  // one non-zero count per apply workgroup
     if (c->mfma)
         hipLaunchKernelGGL(k_sm_apply<1>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, c->Ph, c->Pl,
-                           Tpart);
+                           Tpart, nzpart);
     else
         hipLaunchKernelGGL(k_sm_apply<0>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, nullptr, nullptr,
-                           Tpart);
+                           Tpart, nullptr);
     hipLaunchKernelGGL(k_T_final, dim3((c->Sp + 255) / 256), dim3(256), 0, st, Tpart, ncb, S, c->Sp, c->T);
+    if (c->mfma) hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, nzpart, (int)(ga.x * ga.y), c->scal + SC_NNZ);
     return hipGetLastError();
 }
 

@@ -59,6 +59,11 @@ enum { EMC_PREC_FP32 = 0, EMC_PREC_BF16X2 = 1, EMC_PREC_BF16X3 = 2 };
  *            accumulated per brick in LDS and flushed once (no contended global atomics). */
 enum { EMC_INSERT_ATOMIC = 0, EMC_INSERT_FUSED = 1, EMC_INSERT_BINNED = 2 };
 
+/* How the MFMA path forms U = P^T.K (emc_options.mstep).  SPARSE sums, per rotation, only the
+ * patterns whose responsibility is non-zero (the same sums up to their order); AUTO uses it when
+ * at most 2 % of the responsibilities are non-zero.  FUSED insertion always uses DENSE. */
+enum { EMC_MSTEP_AUTO = 0, EMC_MSTEP_DENSE = 1, EMC_MSTEP_SPARSE = 2 };
+
 typedef struct emc_dims {
     int32_t model_side;  /* L: model L^3, patterns L x L (configs.h:14 model_side) */
     int32_t n_rotations; /* S (rotations.c:457 return value) */
@@ -81,7 +86,11 @@ typedef struct emc_options {
     int32_t mfma_tile;       /* MFMA contraction tiling: 0 = 256-row tiles (default), 1 = 128x128 */
     int32_t count_bytes;     /* counts operand of the 256-row MFMA kernels: 0 = one byte per count when
                                 every count <= 255 (exact; widened to BF16 in registers), 2 = BF16 */
-    int32_t reserved[5];
+    int32_t mstep;           /* MFMA M-step sum P^T.K: EMC_MSTEP_AUTO (default) chooses on the device,
+                                from the E-step's count of non-zero responsibilities, between the
+                                dense contraction and the sparse one (only the non-zero P); or
+                                EMC_MSTEP_DENSE / EMC_MSTEP_SPARSE */
+    int32_t reserved[4];
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone

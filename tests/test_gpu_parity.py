@@ -295,3 +295,38 @@ def test_byte_counts_equal_bf16_counts(small):
             outs.append((R, e.get_model()[0]))
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.allclose(outs[0][1], outs[1][1], rtol=1e-5, atol=1e-7)  # insertion order of float atomics varies
+
+
+@pytest.mark.parametrize("counts", ["bytes", "bf16", "bf16_split"])
+def test_sparse_mstep_equals_dense(orc, small, counts):
+    """The sparse M-step (only the non-zero responsibilities of each rotation) gives the dense
+    contraction's model up to the order of the sum over patterns, for byte counts, BF16 counts and
+    split BF16 counts (> 256); and the oracle's model."""
+    log(f"[sparse M-step] counts={counts}")
+    d = dict(small)
+    K = d["K"][:400].copy()
+    kw = {}
+    if counts == "bf16":
+        kw["count_bytes"] = 2
+    elif counts == "bf16_split":
+        K = K * 40  # counts above 256 need a second exact BF16 term
+        assert K.max() > 256
+    d["K"] = K
+    engines = {mode: engine(d, "bf16x2", I=400, mstep=mode, blur_sigma=0.5, **kw) for mode in ("dense", "sparse", "auto")}
+    try:
+        m = d["m0"].astype(np.float32)
+        for it in range(2):  # every form starts each iteration from the same model
+            out = {}
+            for mode, e in engines.items():
+                e.set_model(m)
+                st = e.iterate(it)
+                out[mode] = e.get_model()[0], st
+            for mode in ("sparse", "auto"):
+                rel = float(np.linalg.norm(out[mode][0] - out["dense"][0]) / np.linalg.norm(out["dense"][0]))
+                dw = abs(out[mode][1]["weight_sum"] - out["dense"][1]["weight_sum"]) / out["dense"][1]["weight_sum"]
+                log(f"   it {it} {mode}: model rel L2 vs dense {rel:.2e}, weight sum rel {dw:.2e}")
+                assert rel < 1e-6 and dw < 1e-6
+            m = out["dense"][0]
+    finally:
+        for e in engines.values():
+            e.close()

@@ -115,11 +115,13 @@ def test_features_are_pooled_log1p(d):
     assert np.allclose(raw, ref, rtol=1e-6, atol=1e-6)
 
 
-def test_grouped_iteration_with_every_cell_equals_full_search(d):
+@pytest.mark.parametrize("mstep", ["dense", "sparse", "auto"])
+def test_grouped_iteration_with_every_cell_equals_full_search(d, mstep):
     """top_k = all 60 cells: the grouped path scores every (rotation, pattern) pair with the same
-    operands and k order as the dense path, so the iteration reproduces the full search."""
-    log("[grouped == full] top_k = C = 60")
-    with engine(d) as a, engine(d) as b:
+    operands and k order as the dense path, so the iteration reproduces the full search; with the
+    dense and the sparse M-step forms."""
+    log(f"[grouped == full] top_k = C = 60, M-step {mstep}")
+    with engine(d, mstep=mstep) as a, engine(d, mstep=mstep) as b:
         a.enable_proposal(d["cq"], top_k=60, warmup_full=1, full_every=1000, epochs=1, hidden=64, batch=64)
         sa0, sb0 = a.iterate(0), b.iterate(0)
         assert sa0["search"] == "full"
