@@ -43,16 +43,17 @@ def traffic_from_profile(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
-def run_engine(d, prec, steps, warmup, insert="binned", mfma_tile=0):
+def run_engine(d, prec, steps, warmup, insert="binned", mfma_tile=0, **extra):
     import emc_amd
     L = d["L"]
     S, I = d["q"].shape[0], d["K"].shape[0]
     t = time.perf_counter()
     e = emc_amd.Engine(L, S, I, precision=prec, insert=insert, blur_sigma=d["cfg"]["blur"], timing=1,
-                       mfma_tile=mfma_tile)
+                       mfma_tile=mfma_tile, **extra)
     e.setup(d["q"], d["w"], d["coords"], d["mask"], d["K"], d["model0"])
     e.normalize_model()
-    log(f"[bench] {prec}: setup {time.perf_counter() - t:.1f}s")
+    tag = prec + "".join(f" {k}={v}" for k, v in extra.items())
+    log(f"[bench] {tag}: setup {time.perf_counter() - t:.1f}s")
     for it in range(warmup):
         e.iterate(it)
     stats = []
@@ -63,7 +64,7 @@ def run_engine(d, prec, steps, warmup, insert="binned", mfma_tile=0):
     e.close()
     ms = (t1 - t0) * 1000.0 / steps
     stage = {k: float(np.mean([s["stage_ms"][k] for s in stats])) for k in stats[0]["stage_ms"]}
-    log(f"[bench] {prec}: {ms:.2f} ms/iter, stages {json.dumps({k: round(v, 3) for k, v in stage.items()})}")
+    log(f"[bench] {tag}: {ms:.2f} ms/iter, stages {json.dumps({k: round(v, 3) for k, v in stage.items()})}")
     return ms, stage, stats[-1]
 
 
@@ -333,6 +334,9 @@ def main():
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
         fp32 = {"value": 1000.0 / ms32, "ms_per_step": ms32, "stage_ms": st32,
                 "logl_gemm_tflops": 2.0 * I * S * Pa / (st32["logl_gemm"] * 1e-3) / 1e12}
+        # the reference's own emc.conf choices: chi^2 ("poisson") with known_intensity = false
+        mss, sts, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic", diff="poisson", scaling=1)
+        fp32["chi2_scaled"] = {"value": 1000.0 / mss, "ms_per_step": mss, "stage_ms": sts}
 
     variants = {}
     if args.variants:

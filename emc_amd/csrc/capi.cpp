@@ -200,7 +200,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->K8); dfree(c->KT8);
     dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
-    dfree(c->A0); dfree(c->A1); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
+    dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
     dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
@@ -223,11 +223,13 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad dimensions");
     if (opt.diff < 0 || opt.diff > 3 || opt.interp < 0 || opt.interp > 1 || opt.precision < 0 || opt.precision > 2 ||
         opt.insert < 0 || opt.insert > 2 || (opt.count_bytes != 0 && opt.count_bytes != 2) || opt.mfma_tile < 0 ||
-        opt.mstep < 0 || opt.mstep > 2 ||
+        opt.mstep < 0 || opt.mstep > 2 || opt.scaling < 0 || opt.scaling > 1 ||
         opt.mfma_tile > 1)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
     if (opt.precision != EMC_PREC_FP32 && opt.diff != EMC_DIFF_TRUE_POISSON)
         return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: MFMA precision modes need diff = TRUE_POISSON");
+    if (opt.scaling && (opt.precision != EMC_PREC_FP32 || opt.diff == EMC_DIFF_TRUE_POISSON))
+        return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: intensity scaling needs precision FP32 and a chi^2 diff");
     if (opt.sigma_half_life == 0 && opt.diff != EMC_DIFF_TRUE_POISSON)
         return fail(nullptr, EMC_E_INVALID, "emc_create: sigma_half_life must be non-zero");
 
@@ -284,6 +286,14 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK) A(dalloc(&c->KT32, Ip * PpMax), "KT32");
         if (st == EMC_OK) A(dalloc(&c->A0, cp * PpMax), "A0");
         if (st == EMC_OK && opt.diff != EMC_DIFF_TRUE_POISSON) A(dalloc(&c->A1, cp * PpMax), "A1");
+        if (opt.scaling) {
+            if (st == EMC_OK) A(dalloc(&c->Am, cp * PpMax), "Am");
+            if (st == EMC_OK) A(dalloc(&c->AWm, cp * PpMax), "AWm");
+            if (st == EMC_OK) A(dalloc(&c->K2, Ip * PpMax), "K2");
+            if (st == EMC_OK) A(dalloc(&c->iphi, Sp * Ip), "iphi");
+            if (st == EMC_OK) A(dalloc(&c->Pphi, Sp * Ip), "Pphi");
+            if (st == EMC_OK) A(launch_fill_f32(c->iphi, 1.0f, Sp * Ip, nullptr), "iphi");
+        }
     }
     if (st == EMC_OK) A(dalloc(&c->colmean, PpMax), "colmean");
     if (st == EMC_OK) A(dalloc(&c->model, N3), "model");
@@ -464,6 +474,7 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
             }
         HIP_TRY(c, hipMemcpy(c->K32, k32.data(), sizeof(float) * k32.size(), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->KT32, kt.data(), sizeof(float) * kt.size(), hipMemcpyHostToDevice));
+        if (c->opt.scaling) HIP_TRY(c, launch_square_counts(c, c->stream));
     }
     HIP_TRY(c, hipMemcpy(c->sumK, sumK.data(), sizeof(double) * Ip, hipMemcpyHostToDevice));
     c->sumK_total = total;
@@ -637,6 +648,7 @@ static emc_status logl_enqueue(emc_ctx *c, float sigma)
             gev_rec(c, 0, k, 1);
         } else {
             HIP_TRY(c, launch_slice_f32(c, s0, cs, st));
+            if (c->opt.scaling) HIP_TRY(c, launch_scaling_f32(c, s0, cs, st));
             gev_rec(c, 0, k, 0);
             HIP_TRY(c, launch_logl_f32(c, s0, cs, st));
             gev_rec(c, 0, k, 1);
@@ -744,6 +756,7 @@ static emc_status mstep_enqueue(emc_ctx *c)
     const int mode = fused || c->Pp > kSparseMaxPp ? EMC_MSTEP_DENSE : c->opt.mstep;
     const double thr = 0.02 * (double)c->S * (double)c->I;
     const double *gate = c->scal + SC_NNZ;
+    if (!c->mfma && c->opt.scaling) HIP_TRY(c, launch_pscale_f32(c, st));  // the M-step sums K P / phi
     for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
         const int cs = std::min(c->chunk, c->S - s0);
         const int k = s0 / c->chunk;

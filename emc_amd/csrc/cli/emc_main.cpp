@@ -88,6 +88,11 @@ int main(int argc, char **argv)
     opt.sigma_half_life = cfg.sigma_half_life;
     opt.blur_sigma = cfg.model_blur;
     opt.fluence = cfg.fluence;
+    // known_intensity = false: per (rotation, pattern) scaling of the chi^2 kinds (main.cpp:283-295).
+    // The true Poisson logL carries its own global fluence and takes no per-pattern factor.
+    opt.scaling = !cfg.known_intensity && cfg.diff != EMC_DIFF_TRUE_POISSON;
+    if (!cfg.known_intensity && !opt.scaling)
+        std::fprintf(stderr, "emc: known_intensity = false has no effect with diff = \"true_poisson\" (global fluence)\n");
     opt.timing = 1;
     emc_ctx *ctx = nullptr;
     if ((st = emc_create(&dims, &opt, &ctx)) != EMC_OK) return die("emc_create", st, nullptr);

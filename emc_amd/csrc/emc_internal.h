@@ -156,6 +156,12 @@ struct emc_ctx {
     float *rscale = nullptr;  // [Sp] chi^2 factor 1/(2 n_s sigma^2)
     double *cst = nullptr;    // [Ip] per-pattern constant removed from the stored logits
     float *A0 = nullptr, *A1 = nullptr;                            // [chunkp][Pp] FP32 slices
+    // intensity scaling (options.scaling): the two operands of its sums, counts squared,
+    // 1/phi per (rotation, pattern) and the M-step's operand P/phi
+    float *Am = nullptr, *AWm = nullptr;  // [chunkp][Pp] 1[W > 1e-10], W 1[W > 1e-10]
+    float *K2 = nullptr;                  // [Ip][Pp] K^2
+    float *iphi = nullptr;                // [Sp][Ip] (1 until the first likelihood pass)
+    float *Pphi = nullptr;                // [Sp][Ip]
     emc::bf16_t *Lh = nullptr, *Lm = nullptr, *Ll = nullptr;       // [chunkp][Pp] BF16 log W terms
     // responsibilities
     float *R = nullptr;                                // [Sp][Ip] logits (P in place on FP32 path)
@@ -200,9 +206,17 @@ hipError_t launch_slice_bf16(const emc_ctx *c, int s0, int cs, hipStream_t st);
 // Full-detector slice values of rotations [s0, s0+cs) into out [cs][P] (-1 invalid/masked).
 hipError_t launch_expand(const emc_ctx *c, int s0, int cs, float *out, hipStream_t st);
 
-// FP32 contractions (plain path).
+// FP32 contractions (plain path).  With options.scaling the logits use iphi (launch_scaling_f32
+// first) and the M-step reads P/phi from iphi (launch_pscale_f32 after the responsibilities).
 hipError_t launch_logl_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
 hipError_t launch_mstep_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
+// 1/phi of rotations [s0, s0+cs) from the chunk's FP32 slices: sum K^2 and sum K W over
+// {W > 1e-10} as two contractions over pixels, then the quotient (1 where either sum is 0).
+hipError_t launch_scaling_f32(const emc_ctx *c, int s0, int cs, hipStream_t st);
+// Pphi = P * iphi for all rotations (P in R).
+hipError_t launch_pscale_f32(const emc_ctx *c, hipStream_t st);
+// K2 = K32^2 (after the counts are set).
+hipError_t launch_square_counts(const emc_ctx *c, hipStream_t st);
 // MFMA contractions.
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st);
 // The dense M-step returns at once when gate is non-null and *gate <= gate_thr.

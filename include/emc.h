@@ -90,7 +90,14 @@ typedef struct emc_options {
                                 from the E-step's count of non-zero responsibilities, between the
                                 dense contraction and the sparse one (only the non-zero P); or
                                 EMC_MSTEP_DENSE / EMC_MSTEP_SPARSE */
-    int32_t reserved[4];
+    int32_t scaling;         /* 1 = per-(rotation, pattern) intensity scaling, the reference's
+                                known_intensity = false (main.cpp:283-295): phi[s,i] =
+                                sum K^2 / sum K W over {mask, W > 1e-10, K > 1e-10}
+                                (emc_cuda.cu:438-454, 516-532), the chi^2 kinds compare W with K/phi
+                                and the M-step sums K P / phi (emc_atomic.cu:106-108); both sums
+                                of phi are contractions over pixels.  FP32 chi^2 kinds only.
+                                0 = phi = 1 (known_intensity = true) */
+    int32_t reserved[3];
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone

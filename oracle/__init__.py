@@ -54,6 +54,8 @@ def lib():
         L.orc_logl.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                f64p, C.c_void_p, C.c_double, f64p, C.c_void_p]
         L.orc_logl.restype = None
+        L.orc_scaling.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, f64p]
+        L.orc_scaling.restype = None
         L.orc_fluence.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int]
         L.orc_fluence.restype = C.c_double
         L.orc_softmax.argtypes = [f64p, C.c_int, C.c_int, f64p]
@@ -73,7 +75,7 @@ def lib():
         L.orc_sigma.restype = C.c_double
         L.orc_iterate.argtypes = [f64p, C.c_int, f32p, f64p, C.c_int, f64p, i32p, C.c_int, f32p, f32p,
                                   f32p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
-                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.orc_iterate.restype = C.c_double
         _lib = L
     return _lib
@@ -144,6 +146,18 @@ def fluence(W, K, mask):
     I = K.shape[0]
     return lib().orc_fluence(np.ascontiguousarray(W, np.float64), np.ascontiguousarray(K, np.float64),
                              np.ascontiguousarray(mask, np.int32), S, I, P)
+
+
+def scaling(W, K, mask):
+    """Per (rotation, pattern) intensity scaling phi [S, I] of known_intensity = 0
+    (emc_cuda.cu:438-455, 516-531): sum K^2 / sum K W over unmasked pixels with W, K > 1e-10."""
+    W = np.ascontiguousarray(W, np.float64)
+    K = np.ascontiguousarray(K, np.float64)
+    S, P = W.shape
+    I = K.shape[0]
+    phi = np.zeros((S, I), np.float64)
+    lib().orc_scaling(W, K, np.ascontiguousarray(mask, np.int32), S, I, P, phi)
+    return phi
 
 
 def logl(W, K, mask, wrot, diff="true_poisson", sigma=1.0, phi=None, phi0=1.0):
@@ -217,7 +231,7 @@ def sigma_at(iteration, s_start, s_final, half_life):
 
 
 def iterate(model, L, q, w, K, mask, coords, interp="trilinear", diff="true_poisson", sigma=1.0,
-            fluence_value=0.0, blur_sigma=0.0, want_P=False):
+            fluence_value=0.0, blur_sigma=0.0, want_P=False, scaling=False):
     """One EMC iteration (main.cpp:239-480 order).  Returns dict with model, Wt, lse, total, ..."""
     model = np.array(model, np.float64, copy=True).ravel()
     q = np.ascontiguousarray(q, np.float32).reshape(-1, 4)
@@ -231,7 +245,8 @@ def iterate(model, L, q, w, K, mask, coords, interp="trilinear", diff="true_pois
     stats = np.zeros(2, np.float64)
     tot = lib().orc_iterate(model, L, q, np.ascontiguousarray(w, np.float64), S, K,
                             np.ascontiguousarray(mask, np.int32), I, x, y, z, INTERP[interp],
-                            DIFF[diff], sigma, fluence_value, blur_sigma, _ptr(Wt), _ptr(Pm),
+                            DIFF[diff], sigma, fluence_value, blur_sigma, int(bool(scaling)),
+                            _ptr(Wt), _ptr(Pm),
                             _ptr(lse), _ptr(stats))
     return {"model": model, "Wt": Wt, "lse": lse, "total": tot, "P": Pm, "phi0": stats[0],
             "avg": stats[1]}

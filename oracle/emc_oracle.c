@@ -457,6 +457,35 @@ ORC_API void orc_logl(const double *W, const double *K, const int32_t *mask, int
     }
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* a5b: per (rotation, pattern) intensity scaling for known_intensity = 0 (main.cpp:283-295,      */
+/* emc_cuda.cu:438-455 calculate_scaling_absolute, 516-531 update_scaling_full_kernel).           */
+/*   phi[s, i] = sum K^2 / sum K W over {p : mask[p] != 0, W[s,p] > 1e-10, K[i,p] > 1e-10}       */
+/* Exact double sums.  The reference accumulates each thread's sum K W in an int (the float      */
+/* product is truncated on every add); this restatement keeps the real-valued sum.  phi = 1 when */
+/* either sum is zero (the reference produces 0/0 or x/0 there).                                 */
+/* Pinned by: closed form for one-pixel and proportional patterns (K = c W gives phi = c) and     */
+/* pure-Python brute force (tests).                                                              */
+/* ------------------------------------------------------------------------------------------ */
+ORC_API void orc_scaling(const double *W, const double *K, const int32_t *mask, int S, int I, int P,
+                         double *phi)
+{
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s = 0; s < S; s++) {
+        const double *Ws = W + (size_t)s * P;
+        for (int i = 0; i < I; i++) {
+            const double *Ki = K + (size_t)i * P;
+            double kk = 0.0, kw = 0.0;
+            for (int p = 0; p < P; p++)
+                if (mask[p] != 0 && Ws[p] > 1e-10 && Ki[p] > 1e-10) {
+                    kk += Ki[p] * Ki[p];
+                    kw += Ki[p] * Ws[p];
+                }
+            phi[(size_t)s * I + i] = (kk > 0.0 && kw > 0.0) ? kk / kw : 1.0;
+        }
+    }
+}
+
 /* Global fluence for true Poisson: mean photons per pattern over mean slice sum (both over
  * unmasked pixels; slice sum over V_s).  No per-pattern scaling. */
 ORC_API double orc_fluence(const double *W, const double *K, const int32_t *mask, int S, int I,
@@ -679,7 +708,8 @@ ORC_API double orc_sigma(int iteration, double s_start, double s_final, int half
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/* a16: one full iteration (main.cpp:239-480 order of operations), FP64.                        */
+/* a16: one full iteration (main.cpp:239-480 order of operations), FP64.  scaling != 0 applies  */
+/* the per (s, i) intensity scaling of known_intensity = 0 to the chi^2 logits and the M-step.  */
 /* model (in/out, L^3), Wt_out (L^3, may be NULL), P_out ([S][I], may be NULL), lse (I, may be  */
 /* NULL).  Returns total logL (sum_i lse_i).  stats (may be NULL): [0]=phi0 [1]=model average    */
 /* before normalisation.                                                                         */
@@ -687,8 +717,8 @@ ORC_API double orc_sigma(int iteration, double s_start, double s_final, int half
 ORC_API double orc_iterate(double *model, int L, const float *quat, const double *wrot, int S,
                            const double *K, const int32_t *mask, int I, const float *X,
                            const float *Y, const float *Z, int interp, int diff, double sigma,
-                           double fluence, double blur_sigma, double *Wt_out, double *P_out,
-                           double *lse_out, double *stats)
+                           double fluence, double blur_sigma, int scaling, double *Wt_out,
+                           double *P_out, double *lse_out, double *stats)
 {
     const int P = L * L;
     const size_t N3 = (size_t)L * L * L;
@@ -700,9 +730,15 @@ ORC_API double orc_iterate(double *model, int L, const float *quat, const double
     orc_slices(model, L, quat, S, X, Y, Z, P, interp, W);
     double phi0 = 1.0;
     if (diff == 3) phi0 = fluence > 0.0 ? fluence : orc_fluence(W, K, mask, S, I, P);
-    orc_logl(W, K, mask, S, I, P, diff, sigma, wrot, NULL, phi0, R, NULL);
+    double *phi = NULL;
+    if (scaling) {
+        phi = (double *)malloc(sizeof(double) * (size_t)S * I);
+        orc_scaling(W, K, mask, S, I, P, phi);
+    }
+    orc_logl(W, K, mask, S, I, P, diff, sigma, wrot, phi, phi0, R, NULL);
     double total = orc_softmax(R, S, I, lse);
-    orc_mstep(R, K, mask, S, I, P, NULL, quat, X, Y, Z, L, interp, Mnum, Wt, NULL);
+    orc_mstep(R, K, mask, S, I, P, phi, quat, X, Y, Z, L, interp, Mnum, Wt, NULL);
+    free(phi);
     orc_compress(model, Mnum, Wt, (long)N3);
     if (blur_sigma > 0.0) orc_blur(model, L, blur_sigma);
     double avg = 0.0;

@@ -69,3 +69,20 @@ def test_cli_with_proposal(tmp_path):
     assert [l["search"] for l in lines][:3] == ["full"] * 3
     assert all(np.isfinite(l["total_logl"]) for l in lines)
     assert lines[3]["search"] in ("proposal", "full")  # full when too many patterns fall back
+
+
+@pytest.mark.gpu
+def test_cli_reference_default_scaling_run(tmp_path):
+    """The reference's own emc.conf choices: diff_type = "poisson" (chi^2) with known_intensity =
+    false, so every (rotation, pattern) pair gets its intensity scaling (main.cpp:283-295)."""
+    print("[cli] synthetic 24^3 run, chi^2 with intensity scaling", flush=True)
+    p = tmp_path / "emc.conf"
+    p.write_text(CONF + 'diff_type = "poisson";\nknown_intensity = false;\n'
+                 "sigma_start = 0.4; sigma_final = 0.12; sigma_half_life = 25;\n")
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    assert [l["iteration"] for l in lines] == [0, 1]
+    assert all(np.isfinite(l["total_logl"]) for l in lines)
+    m = np.fromfile(tmp_path / "output" / "model_0001.bin", np.float32)
+    assert abs(m[m > 0].mean() - 1.0) < 1e-4

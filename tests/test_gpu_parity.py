@@ -83,6 +83,35 @@ def test_logl_fp32_matches_oracle(orc, small, diff, interp):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("diff", ["poisson", "absolute", "relative"])
+def test_scaled_logl_fp32_matches_oracle(orc, small, diff):
+    """known_intensity = 0: per (rotation, pattern) scaling phi = sum K^2 / sum K W (emc_cuda.cu:438-455)
+    divides the counts in the chi^2 logits (emc_cuda.cu:160-216)."""
+    log(f"[logl scaled] diff={diff}")
+    d = small
+    S, I = 700, 300
+    sigma = 0.3
+    with engine(d, "fp32", S=S, I=I, diff=diff, scaling=1, slice_chunk=256) as e:
+        e.logl(sigma)
+        R = e.get_logl()
+    M32 = d["m0"].astype(np.float32).astype(np.float64)
+    W = orc.slices(M32, d["L"], d["q"][:S], d["coords"], "trilinear")
+    K = d["K"][:I].astype(np.float64)
+    phi = orc.scaling(W, K, d["mask"])
+    Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), diff, sigma=sigma, phi=phi)
+    R1, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), diff, sigma=sigma)
+    err = rel_logl(R, Ro)
+    log(f"   rel logL err {err:.2e} (scaling moves the logits by {rel_logl(R1, Ro):.2e})")
+    assert err < 1e-5
+    assert rel_logl(R1, Ro) > 100 * err  # the scaled logits are not the unscaled ones
+
+
+def test_scaling_rejected_where_unsupported(small):
+    for kw in ({"precision": "bf16x2", "diff": "poisson"}, {"precision": "fp32", "diff": "true_poisson"}):
+        with pytest.raises(emc_amd.EmcError):
+            emc_amd.Engine(small["L"], 100, 50, scaling=1, **kw)
+
+
 @pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
 def test_logl_mfma_matches_oracle(orc, small, prec):
     log(f"[logl mfma] prec={prec}")
@@ -178,6 +207,34 @@ def test_full_iterations_match_oracle(orc, small, prec):
             P = e.get_resp()
             ref = orc.iterate(m.astype(np.float64), L, d["q"], w64, K, d["mask"], d["coords"], "trilinear",
                               "true_poisson", blur_sigma=0.5, want_P=True)
+            mt = metrics(None, None, P, ref["P"], M, ref["model"])
+            rel_tot = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
+            log(f"   it {it}: {mt} total logL rel {rel_tot:.2e}")
+            assert rel_tot < 1e-5
+            assert mt["resp_l1_max"] < 5e-3
+            assert mt["model_rel_l2"] < 1e-4
+            m = ref["model"].astype(np.float32)
+
+
+def test_scaled_iterations_match_oracle(orc, small):
+    """Two iterations with known_intensity = 0 and the reference's chi^2 ("poisson") diff: the
+    scaling enters the logits and the M-step sums K P / phi (emc_atomic.cu:107-108)."""
+    log("[iterate scaled]")
+    d = small
+    L = d["L"]
+    S, I = d["q"].shape[0], d["K"].shape[0]
+    K = d["K"].astype(np.float64)
+    w64 = d["w"].astype(np.float64)
+    m = d["m0"].astype(np.float32)
+    sig = 0.5
+    with engine(d, "fp32", diff="poisson", scaling=1, sigma_start=sig, sigma_final=sig, blur_sigma=0.5) as e:
+        for it in range(2):
+            e.set_model(m)
+            st = e.iterate(it)
+            M, _ = e.get_model()
+            P = e.get_resp()
+            ref = orc.iterate(m.astype(np.float64), L, d["q"], w64, K, d["mask"], d["coords"], "trilinear",
+                              "poisson", sigma=sig, blur_sigma=0.5, want_P=True, scaling=True)
             mt = metrics(None, None, P, ref["P"], M, ref["model"])
             rel_tot = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
             log(f"   it {it}: {mt} total logL rel {rel_tot:.2e}")

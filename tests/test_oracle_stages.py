@@ -141,6 +141,52 @@ def test_chi2_kinds_closed_forms(orc, kind):
     assert abs(R[0, 1] - (np.log(0.2) - D / (2 * P * sigma ** 2))) < 1e-10
 
 
+def test_scaling_closed_forms_and_brute_force(orc):
+    """phi[s, i] = sum K^2 / sum K W over unmasked pixels with W, K > 1e-10 (emc_cuda.cu:438-455)."""
+    rng = np.random.default_rng(17)
+    S, P = 3, 24
+    W = rng.uniform(0.1, 3.0, size=(S, P))
+    W[1, :5] = 0.0  # excluded pixels of slice 1
+    mask = np.ones(P, np.int32)
+    mask[-3:] = 0
+    c = np.array([0.5, 2.0, 7.25])
+    K = c[:, None] * W[0][None]  # pattern i is c_i times slice 0
+    K[2, 7] = 0.0  # a zero count drops its pixel from both sums
+    phi = orc.scaling(W, K, mask)
+    assert np.allclose(phi[0], c, rtol=1e-13)
+    one = np.zeros((1, P))
+    one[0, 4] = 3.0  # one counted pixel: phi = K/W there
+    assert np.isclose(orc.scaling(W, one, mask)[0, 0], 3.0 / W[0, 4])
+    assert orc.scaling(W, one, mask)[1, 0] == 1.0  # W = 0 there: no pixel, phi := 1
+    Kr = rng.poisson(1.5, size=(5, P)).astype(np.float64)
+    phr = orc.scaling(W, Kr, mask)
+    for s in range(S):
+        for i in range(5):
+            num = den = 0.0
+            for p in range(P):
+                if mask[p] and W[s, p] > 1e-10 and Kr[i, p] > 1e-10:
+                    num += Kr[i, p] ** 2
+                    den += Kr[i, p] * W[s, p]
+            assert abs(phr[s, i] - (num / den if num > 0 and den > 0 else 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["poisson", "absolute"])
+def test_scaled_copy_of_a_slice_is_a_perfect_match(orc, kind):
+    """With phi from the scaling step, a pattern that is c times slice s has K/phi = W: D = 0."""
+    rng = np.random.default_rng(18)
+    S, P = 4, 30
+    W = rng.uniform(0.5, 2.0, size=(S, P))
+    mask = np.ones(P, np.int32)
+    K = np.stack([3.0 * W[2], 0.4 * W[1]])
+    w = np.full(S, 0.25)
+    phi = orc.scaling(W, K, mask)
+    R, _ = orc.logl(W, K, mask, w, kind, sigma=0.2, phi=phi)
+    assert abs(R[2, 0] - np.log(0.25)) < 1e-12 and abs(R[1, 1] - np.log(0.25)) < 1e-12
+    assert R[2, 0] == R[:, 0].max() and R[1, 1] == R[:, 1].max()
+    R1, _ = orc.logl(W, K, mask, w, kind, sigma=0.2)  # unscaled, the copies no longer match
+    assert R1[2, 0] < np.log(0.25) - 1.0
+
+
 def test_softmax_matches_logsumexp(orc):
     rng = np.random.default_rng(8)
     R = rng.normal(scale=50.0, size=(300, 17)) - 1e4
@@ -167,6 +213,8 @@ def test_mstep_one_hot_reinserts_pattern_and_conserves(orc):
     assert np.allclose(vol[L // 2], Kimg)  # insertion bounds include index 0 (emc_atomic.cu:67)
     assert np.allclose(Wt.reshape(L, L, L)[L // 2], mask.reshape(L, L))
     assert T[0] == 1.0
+    Mh, _, _ = orc.mstep(np.ones((1, 1)), K, mask, q, coords, L, "nearest", phi=np.full((1, 1), 4.0))
+    assert np.allclose(Mh, Mnum / 4.0)  # U sums K P / phi (emc_atomic.cu:107-108)
     # general P: sum Mnum = sum_s sum_p U[s,p], sum Wt = sum_s T_s * n_valid_s (trilinear)
     S, I = 4, 6
     q = rand_quats(rng, S)
@@ -244,3 +292,23 @@ def test_iterate_is_the_composition_of_stages(orc):
     assert np.allclose(out["P"], Pm) and np.allclose(out["lse"], lse)
     assert np.allclose(out["model"], Mc) and np.isclose(out["total"], tot)
     assert np.isclose(out["phi0"], phi0)
+
+
+def test_iterate_with_scaling_is_the_composition_of_stages(orc):
+    rng = np.random.default_rng(12)
+    Ls = 12
+    coords = orc.coords_flat(Ls)
+    mask = orc.mask_annulus(Ls, 1, 6)
+    q, w, _ = orc.rotations(2)
+    M = rng.uniform(0.2, 1.0, size=Ls ** 3)
+    K = rng.poisson(1.0, size=(20, Ls * Ls)).astype(np.float64) * mask
+    out = orc.iterate(M, Ls, q, w.astype(np.float64), K, mask, coords, "trilinear", "poisson",
+                      sigma=0.4, want_P=True, scaling=True)
+    W = orc.slices(M, Ls, q, coords, "trilinear")
+    phi = orc.scaling(W, K, mask)
+    R, _ = orc.logl(W, K, mask, w.astype(np.float64), "poisson", sigma=0.4, phi=phi)
+    Pm, lse, tot = orc.softmax(R)
+    Mnum, Wt, T = orc.mstep(Pm, K, mask, q, coords, Ls, "trilinear", phi=phi)
+    Mc, avg, _ = orc.normalize(orc.compress(Mnum, Wt))
+    assert np.allclose(out["P"], Pm) and np.allclose(out["lse"], lse)
+    assert np.allclose(out["model"], Mc) and np.isclose(out["total"], tot)
