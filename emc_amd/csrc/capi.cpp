@@ -200,7 +200,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->K8); dfree(c->KT8);
     dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
-    dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll);
+    dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
     dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
@@ -281,6 +281,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK) A(dalloc(&c->Ph, Sp * Ip), "Ph");
         if (st == EMC_OK) A(dalloc(&c->Pl, Sp * Ip), "Pl");
         if (st == EMC_OK) A(dalloc(&c->colsum, 2 * (int64_t)PpMax), "colsum");
+        if (st == EMC_OK) A(dalloc(&c->lw32, Sp * PpMax), "lw32");
     } else {
         if (st == EMC_OK) A(dalloc(&c->K32, Ip * PpMax), "K32");
         if (st == EMC_OK) A(dalloc(&c->KT32, Ip * PpMax), "KT32");

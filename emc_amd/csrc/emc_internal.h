@@ -163,6 +163,7 @@ struct emc_ctx {
     float *iphi = nullptr;                // [Sp][Ip] (1 until the first likelihood pass)
     float *Pphi = nullptr;                // [Sp][Ip]
     emc::bf16_t *Lh = nullptr, *Lm = nullptr, *Ll = nullptr;       // [chunkp][Pp] BF16 log W terms
+    float *lw32 = nullptr;    // [Sp][Pp] log W from the statistics pass (-inf where W <= 0), MFMA path
     // responsibilities
     float *R = nullptr;                                // [Sp][Ip] logits (P in place on FP32 path)
     emc::bf16_t *Ph = nullptr, *Pl = nullptr;          // [Sp][Ip] BF16 hi/lo P (MFMA path)

@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void k_gsm_apply(const float *__restrict__ Lb,
 // the non-zero responsibilities of its row of the cell's P block (sparse_row.h; entry n of the row
 // is the pattern slot_pat[sbeg[c] + n]).  One workgroup per fine rotation, in cell order; rotations
 // of cells no pattern selected are left alone (their T is 0 and the insertion skips them).
-template <int KT>
+template <int KT, int NPASS>
 __global__ __launch_bounds__(256) void k_gmstep_sparse(const bf16_t *__restrict__ Ph, const bf16_t *__restrict__ Pl,
                                                        GTab g, const void *__restrict__ Kv,
                                                        const bf16_t *__restrict__ Klo, int Pp,
@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void k_gmstep_sparse(const bf16_t *__restrict_
     const int sb = g.sbeg[c], ns = g.sbeg[c + 1] - sb, nsp = g.nsp[c];
     if (ns == 0) return;
     const int64_t row = g.boff[c] + (int64_t)r * nsp;
-    sparse_row<KT>(Ph + row, Pl + row, ns, g.slot_pat + sb, Kv, Klo, Pp, U + (size_t)rot * Pp);
+    sparse_row<KT, NPASS>(Ph + row, Pl + row, ns, g.slot_pat + sb, Kv, Klo, Pp, U + (size_t)rot * Pp);
 }
 
 // The kMaxRecall cells of largest score seen so far, sorted by score (descending; on equal
@@ -524,12 +524,23 @@ hipError_t launch_grouped_mstep(const emc_ctx *c, hipStream_t st)
     const double thr = 0.02 * (double)p->n_blk;
     if (mode != EMC_MSTEP_DENSE) {
         const GTab gs = tables(c, nullptr);
-        if (c->K8)
-            hipLaunchKernelGGL(k_gmstep_sparse<0>, dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K8, nullptr, c->Pp,
-                               gate, thr, c->U);
-        else
-            hipLaunchKernelGGL(k_gmstep_sparse<1>, dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K16, nullptr,
-                               c->Pp, gate, thr, c->U);
+        const bool np1 = c->Pp <= 4096;
+        constexpr int NP = kSparseMaxPp / 4096;
+        if (c->K8) {
+            if (np1)
+                hipLaunchKernelGGL((k_gmstep_sparse<0, 1>), dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K8, nullptr,
+                                   c->Pp, gate, thr, c->U);
+            else
+                hipLaunchKernelGGL((k_gmstep_sparse<0, NP>), dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K8, nullptr,
+                                   c->Pp, gate, thr, c->U);
+        } else {
+            if (np1)
+                hipLaunchKernelGGL((k_gmstep_sparse<1, 1>), dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K16, nullptr,
+                                   c->Pp, gate, thr, c->U);
+            else
+                hipLaunchKernelGGL((k_gmstep_sparse<1, NP>), dim3(c->S), dim3(256), 0, st, c->Ph, c->Pl, gs, c->K16, nullptr,
+                                   c->Pp, gate, thr, c->U);
+        }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -14,7 +14,7 @@
 namespace emc {
 namespace {
 
-template <int KT>
+template <int KT, int NPASS>
 __global__ __launch_bounds__(256) void k_mstep_sparse(const bf16_t *__restrict__ Ph, const bf16_t *__restrict__ Pl,
                                                       int64_t ldp, int s0, int I, const void *__restrict__ Kv,
                                                       const bf16_t *__restrict__ Klo, int Pp,
@@ -23,24 +23,31 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const bf16_t *__restrict__
 {
     if (gate && *gate > gate_thr) return;  // the dense contraction runs instead
     const size_t o = (size_t)(s0 + blockIdx.x) * ldp;
-    sparse_row<KT>(Ph + o, Pl + o, I, nullptr, Kv, Klo, Pp, U + (size_t)blockIdx.x * Pp);
+    sparse_row<KT, NPASS>(Ph + o, Pl + o, I, nullptr, Kv, Klo, Pp, U + (size_t)blockIdx.x * Pp);
 }
 
 }  // namespace
 
+template <int NPASS>
+static void launch_np(const emc_ctx *c, int s0, int cs, const double *gate, double gate_thr, hipStream_t st)
+{
+    const dim3 grid(cs), blk(256);
+    if (c->K8)
+        hipLaunchKernelGGL((k_mstep_sparse<0, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K8,
+                           nullptr, c->Pp, gate, gate_thr, c->U);
+    else if (c->K16lo)
+        hipLaunchKernelGGL((k_mstep_sparse<2, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16,
+                           c->K16lo, c->Pp, gate, gate_thr, c->U);
+    else
+        hipLaunchKernelGGL((k_mstep_sparse<1, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16,
+                           nullptr, c->Pp, gate, gate_thr, c->U);
+}
+
 hipError_t launch_mstep_sparse(const emc_ctx *c, int s0, int cs, const double *gate, double gate_thr, hipStream_t st)
 {
     if (c->Pp > kSparseMaxPp || c->Pp % 16 != 0) return hipErrorInvalidValue;
-    const dim3 grid(cs), blk(256);
-    if (c->K8)
-        hipLaunchKernelGGL(k_mstep_sparse<0>, grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K8, nullptr,
-                           c->Pp, gate, gate_thr, c->U);
-    else if (c->K16lo)
-        hipLaunchKernelGGL(k_mstep_sparse<2>, grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16,
-                           c->K16lo, c->Pp, gate, gate_thr, c->U);
-    else
-        hipLaunchKernelGGL(k_mstep_sparse<1>, grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16, nullptr,
-                           c->Pp, gate, gate_thr, c->U);
+    if (c->Pp <= 4096) launch_np<1>(c, s0, cs, gate, gate_thr, st);
+    else launch_np<kSparseMaxPp / 4096>(c, s0, cs, gate, gate_thr, st);
     return hipGetLastError();
 }
 

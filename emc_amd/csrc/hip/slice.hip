@@ -49,13 +49,14 @@ __global__ void k_fill(T *__restrict__ p, T v, int64_t n)
 // Per-rotation slice sum over V_s = {active j : W > 0}, |V_s|, and per pixel the sum of log W and
 // the number of rotations with a valid sample (colsum[2j], colsum[2j+1]).
 // kStatsG rotations per workgroup; each thread walks pixels j = tid, tid+256, ... and keeps the
-// kStatsG partial sums in registers.
+// kStatsG partial sums in registers.  With lw, the pass also keeps log W ([Sp][Pp], -inf where
+// W <= 0) for the operand split, which then needs no second gather from the model.
 template <int INTERP>
 __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ model, int L,
                                                      const float *__restrict__ rotm, int S,
                                                      const float *__restrict__ axyz, int Pa, int Pp,
                                                      double *__restrict__ sumW, int32_t *__restrict__ ns,
-                                                     double *__restrict__ colsum)
+                                                     double *__restrict__ colsum, float *__restrict__ lw)
 {
     __shared__ float rm[kStatsG][9];
     __shared__ double red_w[4][kStatsG];
@@ -80,12 +81,15 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
         for (int g = 0; g < kStatsG; ++g) {
             if (g < ng) {
                 const float W = sample<INTERP>(model, L, rm[g], X, Y, Z);
+                float l = -INFINITY;
                 if (W > 0.0f) {
+                    l = logf(W);
                     sw[g] += W;
                     cnt[g] += 1;
-                    col += logf(W);
+                    col += l;
                     colv += 1;
                 }
+                if (lw) lw[(size_t)(s0 + g) * Pp + j] = l;
             }
         }
         if (colsum) {
@@ -175,6 +179,49 @@ __global__ __launch_bounds__(256) void k_slice_bf16(const float *__restrict__ mo
             Lm[o] = mi;
             Ll[o] = f2bf(r1 - bf2f(mi));
         }
+    }
+}
+
+// The same operand split from the log W kept by the statistics pass: x = (log W + log phi0, or 0
+// where W <= 0) - colmean[j], the same float operations as k_slice_bf16, so the terms are
+// bit-identical.  Four pixels per thread (Pp is a multiple of 128).
+template <int NT>
+__global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ lw, int s0, int cs, int Pa, int Pp,
+                                                    const float *__restrict__ colmean,
+                                                    const double *__restrict__ scal, bf16_t *__restrict__ Lh,
+                                                    bf16_t *__restrict__ Lm, bf16_t *__restrict__ Ll)
+{
+    const int r = blockIdx.y;
+    const int j0 = (blockIdx.x * 256 + threadIdx.x) * 4;
+    if (j0 >= Pp) return;
+    const bool row_ok = r < cs;
+    const float logphi0 = (float)scal[SC_LOGPHI0];
+    float l[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    if (row_ok) {
+        const float4 v = *reinterpret_cast<const float4 *>(lw + (size_t)(s0 + r) * Pp + j0);
+        l[0] = v.x; l[1] = v.y; l[2] = v.z; l[3] = v.w;
+    }
+    uint16_t h[4], mi[4], lo[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int j = j0 + q;
+        float x = 0.0f;
+        if (row_ok && j < Pa) x = (l[q] > -INFINITY ? l[q] + logphi0 : 0.0f) - colmean[j];
+        h[q] = f2bf(x);
+        const float r1 = x - bf2f(h[q]);
+        mi[q] = f2bf(r1);
+        lo[q] = f2bf(r1 - bf2f(mi[q]));
+    }
+    auto pack = [](const uint16_t *v) {
+        return make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+    };
+    const size_t o = (size_t)r * Pp + j0;
+    *reinterpret_cast<uint2 *>(Lh + o) = pack(h);
+    if (NT == 2) {
+        *reinterpret_cast<uint2 *>(Ll + o) = pack(mi);
+    } else {
+        *reinterpret_cast<uint2 *>(Lm + o) = pack(mi);
+        *reinterpret_cast<uint2 *>(Ll + o) = pack(lo);
     }
 }
 
@@ -308,10 +355,10 @@ hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st
     double *cs = want_colsum ? c->colsum : nullptr;
     if (c->opt.interp == EMC_INTERP_NEAREST)
         hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_NEAREST>, grid, dim3(256), 0, st, c->model, c->L,
-                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs);
+                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs, c->lw32);
     else
         hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_TRILINEAR>, grid, dim3(256), 0, st, c->model, c->L,
-                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs);
+                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs, c->lw32);
     return hipGetLastError();
 }
 
@@ -363,6 +410,16 @@ hipError_t launch_slice_f32(const emc_ctx *c, int s0, int cs, hipStream_t st)
 
 hipError_t launch_slice_bf16(const emc_ctx *c, int s0, int cs, hipStream_t st)
 {
+    if (c->lw32) {
+        const dim3 g((c->Pp / 4 + 255) / 256, c->chunkp);
+        if (c->la_terms == 3)
+            hipLaunchKernelGGL(k_split_bf16<3>, g, dim3(256), 0, st, c->lw32, s0, cs, c->Pa, c->Pp, c->colmean, c->scal,
+                               c->Lh, c->Lm, c->Ll);
+        else
+            hipLaunchKernelGGL(k_split_bf16<2>, g, dim3(256), 0, st, c->lw32, s0, cs, c->Pa, c->Pp, c->colmean, c->scal,
+                               c->Lh, c->Lm, c->Ll);
+        return hipGetLastError();
+    }
     const dim3 grid(c->chunkp), blk(256);
     const bool nn = c->opt.interp == EMC_INTERP_NEAREST;
     if (c->la_terms == 3) {

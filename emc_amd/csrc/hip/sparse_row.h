@@ -2,11 +2,14 @@
 // a row of responsibilities stored as BF16 hi + lo (p_n = hi + lo).  Device code shared by the
 // dense-layout and the per-cell (grouped) M-steps; one 256-thread workgroup per row.
 //
-// The row is scanned 8 entries per thread and step (16-byte loads); the non-zero entries are
-// compacted into LDS in ascending order (wave scans and a block prefix sum keep the order, so the
-// result is deterministic) and, when the list is full or the row ends, their count rows are added
-// to registers, 16 pixels per thread and pass.  Counts as bytes (KT 0), BF16 (KT 1) or BF16
-// hi + lo (KT 2).  FP32 accumulation.
+// The row is scanned 32 entries per thread and step (two 16-byte loads of hi per thread in flight;
+// lo only beside a non-zero hi, which is exact for the split lo = bf16(p - hi)).  A step with no
+// non-zero hi in the whole workgroup ends after one barrier; otherwise its non-zero entries are
+// compacted into an LDS list in ascending order (wave scans and a block prefix sum keep the order,
+// so the result is deterministic), in as many batches as the list needs, and each full list (and
+// the last) has its count rows added to registers, 16 pixels per thread and pass.  Counts as
+// bytes (KT 0), BF16 (KT 1) or BF16 hi + lo (KT 2); NPASS 16-pixel chunks per thread (Pp <=
+// 4096 * NPASS).  FP32 accumulation.
 #pragma once
 
 #include "../emc_internal.h"
@@ -15,16 +18,19 @@ namespace emc {
 
 namespace sparse_row_detail {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-constexpr int kCap = 4096;                  // LDS list entries per batch
-constexpr int kStep = 256 * 8;              // entries scanned per step
-constexpr int kPass = kSparseMaxPp / 4096;  // 16-pixel chunks per thread
+constexpr int kCap = 2048;        // LDS list entries per batch
+constexpr int kV = 4;             // 16-byte loads of hi per thread and step
+constexpr int kPer = 8 * kV;      // consecutive entries per thread and step
+constexpr int kStep = 256 * kPer; // entries scanned per step
 
 __device__ __forceinline__ float half16(const u32x4 &v, int k) { return bf2f((bf16_t)(v[k >> 1] >> (16 * (k & 1)))); }
+__device__ __forceinline__ bool nonzero(const u32x4 &v) { return (v[0] | v[1] | v[2] | v[3]) != 0u; }
 }  // namespace sparse_row_detail
 
 // n entries of the row; pat == nullptr: entry n is pattern n.  Pp is a multiple of 16 and at most
-// kSparseMaxPp; the row holds a multiple of 8 readable entries (padding entries are 0).
-template <int KT>
+// 4096 * NPASS; the row holds a multiple of 8 readable entries (padding entries are 0) and starts
+// 16-byte aligned.
+template <int KT, int NPASS>
 __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const bf16_t *__restrict__ pl, int n,
                                            const int32_t *__restrict__ pat, const void *__restrict__ Kv,
                                            const bf16_t *__restrict__ Klo, int Pp, float *__restrict__ out)
@@ -35,15 +41,15 @@ __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const 
     __shared__ int wsum[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nch = Pp / 16;
-    float acc[kPass][16];
+    float acc[NPASS][16];
 #pragma unroll
-    for (int q = 0; q < kPass; ++q)
+    for (int q = 0; q < NPASS; ++q)
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc[q][k] = 0.0f;
 
     auto accumulate = [&](int cnt) {
 #pragma unroll
-        for (int q = 0; q < kPass; ++q) {
+        for (int q = 0; q < NPASS; ++q) {
             const int pc = tid + q * 256;
             if (pc >= nch) break;
             for (int e = 0; e < cnt; ++e) {
@@ -78,17 +84,26 @@ __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const 
 
     int m = 0;  // entries in the list (the same in every thread)
     for (int i0 = 0; i0 < n; i0 += kStep) {
-        const int ib = i0 + tid * 8;
-        u32x4 h = {0u, 0u, 0u, 0u}, l = {0u, 0u, 0u, 0u};
-        if (ib < n) {
-            h = *reinterpret_cast<const u32x4 *>(ph + ib);
-            l = *reinterpret_cast<const u32x4 *>(pl + ib);
+        const int ib = i0 + tid * kPer;
+        u32x4 h[kV], l[kV];
+        bool any = false;
+#pragma unroll
+        for (int v = 0; v < kV; ++v) {
+            h[v] = u32x4{0u, 0u, 0u, 0u};
+            l[v] = u32x4{0u, 0u, 0u, 0u};
+            if (ib + 8 * v < n) h[v] = *reinterpret_cast<const u32x4 *>(ph + ib + 8 * v);
+            any |= nonzero(h[v]);
         }
+        if (!__syncthreads_or(any)) continue;
         int cnt = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint32_t bits = ((h[k >> 1] | l[k >> 1]) >> (16 * (k & 1))) & 0xffffu;
-            cnt += (bits != 0u && ib + k < n) ? 1 : 0;
+        for (int v = 0; v < kV; ++v) {
+            if (nonzero(h[v])) l[v] = *reinterpret_cast<const u32x4 *>(pl + ib + 8 * v);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t bits = ((h[v][k >> 1] | l[v][k >> 1]) >> (16 * (k & 1))) & 0xffffu;
+                cnt += (bits != 0u && ib + 8 * v + k < n) ? 1 : 0;
+            }
         }
         int incl = cnt;  // inclusive scan over the wave
 #pragma unroll
@@ -101,28 +116,39 @@ __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const 
         int woff = 0;
         for (int w = 0; w < wave; ++w) woff += wsum[w];
         const int step_total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (m + step_total > kCap) {  // block-uniform
-            accumulate(m);
+        const int rank0 = woff + incl - cnt;  // rank of this thread's first entry within the step
+        // the step's entries with rank in [base, base + take) fill the list from position m
+        for (int base = 0;;) {  // block-uniform loop
+            const int take = min(kCap - m, step_total - base);
+            int rank = rank0;
+#pragma unroll
+            for (int v = 0; v < kV; ++v)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint16_t hb = (uint16_t)(h[v][k >> 1] >> (16 * (k & 1)));
+                    const uint16_t lb = (uint16_t)(l[v][k >> 1] >> (16 * (k & 1)));
+                    if ((hb | lb) != 0 && ib + 8 * v + k < n) {
+                        const int q = rank - base;
+                        if (q >= 0 && q < take) {
+                            li[m + q] = pat ? pat[ib + 8 * v + k] : ib + 8 * v + k;
+                            lp[m + q] = bf2f(hb) + bf2f(lb);
+                        }
+                        ++rank;
+                    }
+                }
+            m += take;
+            base += take;
+            __syncthreads();
+            if (base >= step_total) break;
+            accumulate(m);  // the list is full
             __syncthreads();
             m = 0;
         }
-        int pos = m + woff + incl - cnt;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const uint16_t hb = (uint16_t)(h[k >> 1] >> (16 * (k & 1))), lb = (uint16_t)(l[k >> 1] >> (16 * (k & 1)));
-            if ((hb | lb) != 0 && ib + k < n) {
-                li[pos] = pat ? pat[ib + k] : ib + k;
-                lp[pos] = bf2f(hb) + bf2f(lb);
-                ++pos;
-            }
-        }
-        m += step_total;
-        __syncthreads();
     }
     accumulate(m);
 
 #pragma unroll
-    for (int q = 0; q < kPass; ++q) {
+    for (int q = 0; q < NPASS; ++q) {
         const int pc = tid + q * 256;
         if (pc >= nch) break;
 #pragma unroll
