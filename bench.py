@@ -9,7 +9,7 @@ beside it (`fp32`), and the FP64 CPU oracle on a rotation subset is the `cpu_bas
 and 50,000 patterns (64^3 model, 2,000 photons): the same iterations from the same start.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload NAME] [--no-cpu] [--no-fp32]
-                    [--no-proposal] [--proposal-iters 23] [--accuracy] [--variants]
+                    [--no-proposal] [--proposal-iters 48] [--accuracy] [--variants]
 """
 from __future__ import annotations
 
@@ -284,6 +284,24 @@ def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: b
         # steady state over one period of the policy: (full_every - 1) proposal + 1 full iteration
         period = 5
         out["speedup_amortised"] = ref_ms * period / ((period - 1) * float(np.mean(prop_ms)) + float(np.mean(prop_full_ms)))
+        # whole runs after the 3 warm-up iterations: every iteration of the policy (proposal, full
+        # searches with and without retraining) against the full-search run over the same iterations
+        run_ms = float(sum(r["ms"] for r in rows if r["it"] >= 3))
+        ref_run = float(sum(full_ms[3:]))
+        out["speedup_run_after_warmup"] = ref_run / run_ms if run_ms > 0 else None
+        out["full_searches_retrained"] = [r["it"] for r in rows if r["search"] == "full" and r["train_loss"] >= 0]
+        late = [r for r in rows if r["it"] >= iters // 2]
+        late_prop = [r["ms"] for r in late if r["search"] == "proposal"]
+        if late_prop:
+            late_ms = float(sum(r["ms"] for r in late))
+            out["second_half"] = {
+                "iterations": [late[0]["it"], late[-1]["it"]],
+                "proposal_iteration_ms": float(np.mean(late_prop)),
+                "speedup_proposal_iteration": ref_ms / float(np.mean(late_prop)),
+                "speedup_run": float(sum(full_ms[late[0]["it"]:])) / late_ms,
+                "captured_mass_after_proposals": [r["captured_mass"] for k, r in enumerate(rows)
+                                                  if r["it"] >= iters // 2 and r["captured_mass"] >= 0
+                                                  and rows[k - 1]["search"] == "proposal"]}
     return out
 
 
@@ -303,7 +321,7 @@ def main():
     ap.add_argument("--no-proposal", action="store_true",
                     help="skip the learned proposal at 204,960 rotations against the full search")
     ap.add_argument("--proposal-only", action="store_true", help="only the learned-proposal comparison")
-    ap.add_argument("--proposal-iters", type=int, default=23)
+    ap.add_argument("--proposal-iters", type=int, default=48)
     ap.add_argument("--proposal-options", default="{}", help="JSON object of emc_proposal_options overrides")
     ap.add_argument("--diagnose", action="store_true", help="proposal: where the missed responsibility mass went")
     args = ap.parse_args()

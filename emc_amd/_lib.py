@@ -49,7 +49,7 @@ class IterStats(C.Structure):
 class ProposalOptions(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("top_k", "warmup_full", "full_every", "hidden", "epochs", "batch",
                                           "synth_patterns", "refresh_epochs", "recall", "train_patterns")] + \
-               [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "refresh_fraction")] + \
+               [(n, C.c_float) for n in ("min_mass", "lr", "max_fallback", "refresh_fraction", "train_below")] + \
                [("seed", C.c_uint64)]
 
 

@@ -939,8 +939,12 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         // iteration uses: full iterations train on the targets plus those, proposal iterations
         // refresh on those alone
         ev_start(c, EMC_STAGE_TRAIN);
-        if (full) EMC_TRY(proposal_train(c, pr->o.epochs, pr->o.train_patterns != 0, 1.0, &train_loss));
-        else if (pr->o.refresh_epochs > 0)
+        // after the warm-up, a full search whose measured capture is at train_below or above
+        // leaves the network as it is (the settled regime needs no retraining)
+        const bool retrain = pr->iter < pr->o.warmup_full || !(captured >= 0.0) || captured < (double)pr->o.train_below;
+        if (full) {
+            if (retrain) EMC_TRY(proposal_train(c, pr->o.epochs, pr->o.train_patterns != 0, 1.0, &train_loss));
+        } else if (pr->o.refresh_epochs > 0)
             EMC_TRY(proposal_train(c, pr->o.refresh_epochs, false, pr->o.refresh_fraction, &train_loss));
         ev_stop(c, EMC_STAGE_TRAIN);
     }

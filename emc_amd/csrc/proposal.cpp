@@ -460,6 +460,7 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->recall = 8;
     o->train_patterns = 1;
     o->refresh_fraction = 0.25f;
+    o->train_below = 0.9999f;
     o->min_mass = 0.99f;
     o->lr = 1e-3f;
     o->max_fallback = 0.1f;
@@ -626,6 +627,7 @@ extern "C" emc_status emc_proposal_enable(emc_ctx *c, const float *coarse, int32
         !(o.lr > 0.0f) || !(o.min_mass >= 0.0f) || !(o.max_fallback >= 0.0f) || o.synth_patterns < -1 ||
         o.refresh_epochs < 0 || o.refresh_epochs > 1000 || o.recall < 0 || o.recall > kMaxRecall ||
         (o.train_patterns != 0 && o.train_patterns != 1) || !(o.refresh_fraction > 0.0f && o.refresh_fraction <= 1.0f) ||
+        !(o.train_below >= 0.0f && o.train_below <= 1.0f) ||
         o.synth_patterns > (1 << 22))
         return pfail(c, EMC_E_INVALID, "proposal_enable: bad option value");
     P_HIP(c, hipSetDevice(c->opt.device));

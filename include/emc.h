@@ -280,6 +280,10 @@ typedef struct emc_proposal_options {
     float max_fallback;     /* fraction of fully searched patterns above which the iteration
                                runs a full search instead (0.1) */
     float refresh_fraction; /* share of synth_patterns drawn for each refresh pass (0.25) */
+    float train_below;      /* after the warm-up, a full search trains the network only when the
+                               responsibility mass its prediction captured (measured by that
+                               full search) is below this; 1 trains at every full search, 0 never
+                               (0.9999) */
     uint64_t seed;          /* initial weights, shuffling, synthetic patterns */
 } emc_proposal_options;
 

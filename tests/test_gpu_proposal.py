@@ -232,6 +232,24 @@ def test_recall_follows_the_search(d):
         check_top_logit_cells(e, d, e.proposal_recall(), R)
 
 
+@pytest.mark.parametrize("train_below", [0.0, 1.0])
+def test_full_search_retrains_only_below_threshold(d, train_below):
+    """After the warm-up, a full search retrains the network only when the capture it measured is
+    below train_below: never with 0, always with 1; warm-up searches always train."""
+    log(f"[retrain policy] train_below={train_below}")
+    with engine(d) as e:
+        e.enable_proposal(d["cq"], top_k=8, warmup_full=1, full_every=2, epochs=1, hidden=64, batch=64,
+                          train_below=train_below, refresh_epochs=0, max_fallback=1.0)
+        s0 = e.iterate(0)
+        assert s0["search"] == "full" and s0["train_loss"] >= 0  # warm-up
+        s1, s2 = e.iterate(1), e.iterate(2)
+        assert s1["search"] == "proposal" and s2["search"] == "full" and s2["captured_mass"] >= 0
+        log(f"  it 2 captured {s2['captured_mass']:.4f} train_loss {s2['train_loss']:.4f}")
+        assert (s2["train_loss"] >= 0) == (train_below == 1.0)
+        with pytest.raises(emc_amd.EmcError):
+            e.enable_proposal(d["cq"], train_below=1.5)
+
+
 def test_proposal_rejects_bad_use(d):
     log("[proposal errors]")
     with emc_amd.Engine(d["L"], d["q"].shape[0], 64, precision="fp32") as e:

@@ -54,5 +54,6 @@ def test_proposal_defaults():
     assert (o.top_k, o.warmup_full, o.full_every, o.hidden, o.batch) == (8, 3, 5, 512, 256)
     assert abs(o.min_mass - 0.99) < 1e-6 and o.synth_patterns == -1
     assert (o.recall, o.refresh_epochs, o.train_patterns) == (8, 1, 1)
+    assert abs(o.train_below - 0.9999) < 1e-7
     with pytest.raises(KeyError):
         emc_amd.proposal_options(nonsense=1)
