@@ -369,7 +369,7 @@ def main():
     flops = 2.0 * I * S * Pa  # one logical K.logW^T product, counted once (the BF16 split is overhead)
     t_logl = st_mfma["logl_gemm"] * 1e-3
     achieved = flops / t_logl / 1e12
-    roof = {"kernel": "logl_mfma (k_gemm_mfma<2,1,logL>)", "bound": "mfma", "achieved": achieved,
+    roof = {"kernel": "logl_mfma (k_gemm_mfma2<2, 0, 256, true>)", "bound": "mfma", "achieved": achieved,
             "peak": PEAK["bf16_tflops"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16_tflops"],
             "traffic": traffic_from_profile("logl_mfma", args.workload),
             "algorithmic": f"2*I*S*P_act = {flops:.3e} FLOP per launch (one launch per iteration)",
