@@ -15,6 +15,7 @@ from ._lib import (DIFF, INSERT, INTERP, MSTEP, PRECISION, SEARCH, STAGES, Dims,
                    ProposalOptions, check, lib)
 
 __all__ = ["Engine", "EmcError", "rotations", "rotation_count", "coords_flat", "coords_ewald", "mask_annulus",
+           "model_init_radial", "model_init_orientations", "model_read_raw", "normalize_patterns",
            "sigma_schedule", "synth_object", "synth_model", "synth_patterns", "synth_uniform_model",
            "default_options", "device_info", "proposal_options", "coarse_partition"]
 
@@ -107,6 +108,40 @@ def default_options(**kw) -> Options:
             raise KeyError(k)
         setattr(o, k, v)
     return o
+
+
+def model_init_radial(L: int, patterns):
+    """model_input = 3: radial average of the patterns [I][L*L] (emc_model_init_radial)."""
+    K = np.ascontiguousarray(patterns, np.float32).reshape(-1, L * L)
+    m = np.zeros(L ** 3, np.float32)
+    check(lib().emc_model_init_radial(L, K, K.shape[0], m))
+    return m
+
+
+def model_init_orientations(L: int, patterns, mask, coords, seed: int):
+    """model_input = 1: patterns inserted at seeded random orientations; returns (model, q [I][4])."""
+    K = np.ascontiguousarray(patterns, np.float32).reshape(-1, L * L)
+    x, y, z = (np.ascontiguousarray(c, np.float32) for c in coords)
+    q = np.zeros((K.shape[0], 4), np.float32)
+    m = np.zeros(L ** 3, np.float32)
+    check(lib().emc_model_init_orientations(L, K, K.shape[0], np.ascontiguousarray(mask, np.int32), x, y, z, seed,
+                                            q.ctypes.data_as(C.c_void_p), m))
+    return m, q
+
+
+def model_read_raw(path: str, L: int):
+    """model_input = 2: |model| from raw float32 [L^3] (emc_model_read_raw)."""
+    m = np.zeros(L ** 3, np.float32)
+    check(lib().emc_model_read_raw(str(path).encode(), L, m))
+    return m
+
+
+def normalize_patterns(patterns, mask):
+    """normalize_images: each pattern scaled by P / its sum over mask == 1 (emc_normalize_patterns)."""
+    K = np.ascontiguousarray(patterns, np.float32)
+    out = np.zeros_like(K)
+    check(lib().emc_normalize_patterns(K, K.shape[0], K.shape[1], np.ascontiguousarray(mask, np.int32), out))
+    return out
 
 
 def device_info(device: int = 0):

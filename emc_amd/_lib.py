@@ -121,6 +121,11 @@ SIGNATURES = {
     "emc_synth_patterns": (C.c_int, [f32p, C.c_int32, f32p, f32p, f32p, i32p, C.c_int32, C.c_double, C.c_uint64,
                                      i32p, vp]),
     "emc_synth_uniform_model": (C.c_int, [C.c_int32, C.c_uint64, f32p]),
+    "emc_model_init_radial": (C.c_int, [C.c_int32, f32p, C.c_int32, f32p]),
+    "emc_model_init_orientations": (C.c_int, [C.c_int32, f32p, C.c_int32, i32p, f32p, f32p, f32p, C.c_uint64,
+                                              C.c_void_p, f32p]),
+    "emc_model_read_raw": (C.c_int, [C.c_char_p, C.c_int32, f32p]),
+    "emc_normalize_patterns": (C.c_int, [f32p, C.c_int32, C.c_int32, i32p, f32p]),
     "emc_config_defaults": (C.c_int, [C.POINTER(Config)]),
     "emc_config_parse": (C.c_int, [C.c_char_p, C.POINTER(Config)]),
     "emc_config_read": (C.c_int, [C.c_char_p, C.POINTER(Config)]),

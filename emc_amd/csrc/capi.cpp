@@ -941,7 +941,8 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         ev_start(c, EMC_STAGE_TRAIN);
         // after the warm-up, a full search whose measured capture is at train_below or above
         // leaves the network as it is (the settled regime needs no retraining)
-        const bool retrain = pr->iter < pr->o.warmup_full || !(captured >= 0.0) || captured < (double)pr->o.train_below;
+        const bool retrain = pr->iter < pr->o.warmup_full || !(captured >= 0.0) || pr->o.train_below >= 1.0f ||
+                             captured < (double)pr->o.train_below;
         if (full) {
             if (retrain) EMC_TRY(proposal_train(c, pr->o.epochs, pr->o.train_patterns != 0, 1.0, &train_loss));
         } else if (pr->o.refresh_epochs > 0)

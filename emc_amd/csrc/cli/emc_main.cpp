@@ -7,11 +7,17 @@
 // max_iterations EMC iterations.  Per iteration it prints one JSON line (iteration, sigma,
 // total logL, fluence, model average, stage times), appends "iteration rank seconds" to
 // ExeTime.data (file_helper.cpp:119-126) and, when isOutput is set, writes the model and weight
-// as raw little-endian float32 volumes output/model_%04d.bin and output/weight_%04d.bin
-// (the reference writes output/model_%04d.h5; HDF5 output is not part of this build).
+// as raw little-endian float32 volumes output/model_%04d.bin and output/weight_%04d.bin, the model
+// filtered as write_model does (the voxel where weight > 0 and model > 0, else 0;
+// file_helper.cpp:50-65; the reference writes libspimage HDF5, which is not part of this build).
+// With isDebug set, each full-search iteration's responsibilities go to debug/respons_%04d.data,
+// raw float32 [rotations][patterns] (file_helper.cpp:14-27, main.cpp:394-416).
 //
 // Data: image_prefix = "synthetic" generates the workload from the synth_* keys (Gaussian-atom
-// object, flat or Ewald detector, annular mask, Poisson patterns, uniform random initial model).
+// object, flat or Ewald detector, annular mask, Poisson patterns).  normalize_images scales each
+// pattern by P / its masked sum (main.cpp:96-97).  Initial model by model_input
+// (emc_helper.cpp:440-534): 0 uniform random, 1 the patterns inserted at random orientations,
+// 2 read from model_file (raw float32 [L^3]), 3 the patterns' radial average.
 #include <sys/stat.h>
 
 #include <chrono>
@@ -74,14 +80,44 @@ int main(int argc, char **argv)
     if ((st = emc_synth_patterns(truth.data(), L, x.data(), y.data(), z.data(), mask.data(), I, cfg.synth_photons,
                                  2000 + seed, K.data(), nullptr)) != EMC_OK)
         return die("synthetic patterns", st, nullptr);
-    if ((st = emc_synth_uniform_model(L, 3000 + seed, model.data())) != EMC_OK) return die("initial model", st, nullptr);
+    std::vector<float> Kf;
+    if (cfg.normalize_images) {
+        Kf.assign(K.begin(), K.end());
+        if ((st = emc_normalize_patterns(Kf.data(), I, P, mask.data(), Kf.data())) != EMC_OK)
+            return die("normalize_images", st, nullptr);
+    }
+    switch (cfg.model_input) {
+    case 0:
+        st = emc_synth_uniform_model(L, 3000 + seed, model.data());
+        break;
+    case 1:
+    case 3:
+        if (Kf.empty()) Kf.assign(K.begin(), K.end());
+        st = cfg.model_input == 1 ? emc_model_init_orientations(L, Kf.data(), I, mask.data(), x.data(), y.data(),
+                                                                z.data(), 3000 + seed, nullptr, model.data())
+                                  : emc_model_init_radial(L, Kf.data(), I, model.data());
+        if (!cfg.normalize_images) std::vector<float>().swap(Kf);
+        break;
+    case 2:
+        st = emc_model_read_raw(cfg.model_file, L, model.data());
+        break;
+    default:
+        std::fprintf(stderr, "emc: model_input must be 0, 1, 2 or 3\n");
+        return 2;
+    }
+    if (st != EMC_OK) return die("initial model", st, nullptr);
 
     emc_dims dims = {L, S, I, cfg.slice_chunk};
     emc_options opt;
     emc_default_options(&opt);
     opt.diff = cfg.diff;
     opt.interp = cfg.interp;
-    opt.precision = cfg.diff == EMC_DIFF_TRUE_POISSON ? cfg.precision : EMC_PREC_FP32;
+    // the MFMA modes need the true Poisson likelihood and integer counts (exact in BF16); the chi^2
+    // kinds and normalised patterns run on the FP32 path
+    opt.precision = cfg.diff == EMC_DIFF_TRUE_POISSON && Kf.empty() ? cfg.precision : EMC_PREC_FP32;
+    if (opt.precision != cfg.precision)
+        std::fprintf(stderr, "emc: %s runs on the FP32 path\n",
+                     cfg.diff != EMC_DIFF_TRUE_POISSON ? "a chi^2 diff_type" : "normalize_images");
     opt.insert = cfg.insert;
     opt.sigma_start = cfg.sigma_start;
     opt.sigma_final = cfg.sigma_final;
@@ -103,7 +139,8 @@ int main(int argc, char **argv)
     };
     if (guard(emc_set_rotations(ctx, q.data(), w.data()), "set_rotations") &&
         guard(emc_set_coords(ctx, x.data(), y.data(), z.data()), "set_coords") &&
-        guard(emc_set_mask(ctx, mask.data()), "set_mask") && guard(emc_set_patterns_i32(ctx, K.data()), "set_patterns") &&
+        guard(emc_set_mask(ctx, mask.data()), "set_mask") &&
+        guard(Kf.empty() ? emc_set_patterns_i32(ctx, K.data()) : emc_set_patterns_f32(ctx, Kf.data()), "set_patterns") &&
         guard(emc_set_model(ctx, model.data()), "set_model") && guard(emc_normalize_model(ctx, nullptr), "normalize") &&
         (cfg.proposal_coarse_n == 0 || [&] {
             // learned proposal over the cells of the coarse grid
@@ -118,6 +155,8 @@ int main(int argc, char **argv)
         }())) {
         FILE *tf = std::fopen("ExeTime.data", "wb");
         if (cfg.isOutput) mkdir("output", 0755);
+        if (cfg.isDebug) mkdir("debug", 0755);
+        std::vector<float> resp;
         for (int it = 0; it < cfg.max_iterations && rc == 0; ++it) {
             const auto t0 = std::chrono::steady_clock::now();
             emc_iter_stats s;
@@ -137,9 +176,21 @@ int main(int argc, char **argv)
                 std::fprintf(tf, "%d %d   %e \n", it, 0, sec);
                 std::fflush(tf);
             }
+            if (cfg.isDebug && s.search == EMC_SEARCH_FULL) {
+                resp.resize((size_t)S * I);
+                if (!guard(emc_get_resp(ctx, resp.data()), "get_resp")) break;
+                char name[64];
+                std::snprintf(name, sizeof name, "debug/respons_%04d.data", it);
+                if (!write_raw(name, resp.data(), resp.size())) {
+                    std::fprintf(stderr, "emc: cannot write %s\n", name);
+                    rc = 1;
+                }
+            }
             const bool out_now = cfg.isOutput && (!cfg.output_loop || cfg.output_period <= 0 || it % cfg.output_period == 0);
             if (out_now) {
                 if (!guard(emc_get_model(ctx, model.data(), weight.data()), "get_model")) break;
+                for (size_t v = 0; v < N3; ++v)
+                    if (!(weight[v] > 0.0f && model[v] > 0.0f)) model[v] = 0.0f;  // write_model's mask
                 char name[64];
                 std::snprintf(name, sizeof name, "output/model_%04d.bin", it);
                 bool ok = write_raw(name, model.data(), N3);

@@ -239,6 +239,24 @@ emc_status emc_synth_patterns(const float *model, int32_t L, const float *x, con
 /* Uniform random [0,1) model (model_input = 0, emc_helper.cpp:449-456, own RNG). */
 emc_status emc_synth_uniform_model(int32_t L, uint64_t seed, float *model);
 
+/* model_input = 3 (emc_helper.cpp:457-500): radial average of the patterns [I][L*L] over integer
+ * pixel-radius shells, linearly interpolated onto voxel radii < L/2; -1 outside.  L even. */
+emc_status emc_model_init_radial(int32_t L, const float *patterns, int32_t n_patterns, float *model);
+/* model_input = 1 (emc_helper.cpp:501-516 with insert_slice, :80-123): each pattern inserted at a
+ * seeded uniform random orientation (nearest voxel round(L/2 + 0.5 + R x), mask == 1 pixels, weight
+ * 1), then divided by the weight (0 where none).  q_out [I][4] receives the orientations (may be
+ * NULL). */
+emc_status emc_model_init_orientations(int32_t L, const float *patterns, int32_t n_patterns,
+                                       const int32_t *mask, const float *x, const float *y,
+                                       const float *z, uint64_t seed, float *q_out, float *model);
+/* model_input = 2 (emc_helper.cpp:517-531): |model| from a file, here raw float32 [L^3] in voxel
+ * order ((z * L) + y) * L + x (the `emc` driver's output format) instead of libspimage HDF5. */
+emc_status emc_model_read_raw(const char *path, int32_t L, float *model);
+/* normalize_images (main.cpp:96-97, emc_helper.cpp:389-419): each pattern [P] scaled by
+ * P / sum_{mask == 1} of its values (left as it is when that sum is 0).  out may equal patterns. */
+emc_status emc_normalize_patterns(const float *patterns, int32_t n_patterns, int32_t P,
+                                  const int32_t *mask, float *out);
+
 /* ---- learned orientation proposal (no reference counterpart: main.cpp:239-480 always scores
  * every rotation for every pattern) -------------------------------------------------------------
  *

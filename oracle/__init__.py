@@ -54,6 +54,12 @@ def lib():
         L.orc_logl.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                f64p, C.c_void_p, C.c_double, f64p, C.c_void_p]
         L.orc_logl.restype = None
+        L.orc_model_radial.argtypes = [C.c_int, f32p, C.c_int, f64p]
+        L.orc_model_radial.restype = None
+        L.orc_model_orientations.argtypes = [C.c_int, f32p, C.c_int, i32p, f32p, f32p, f32p, f32p, f64p]
+        L.orc_model_orientations.restype = None
+        L.orc_normalize_patterns.argtypes = [f32p, C.c_int, C.c_int, i32p, f64p]
+        L.orc_normalize_patterns.restype = None
         L.orc_scaling.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, f64p]
         L.orc_scaling.restype = None
         L.orc_fluence.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int]
@@ -146,6 +152,32 @@ def fluence(W, K, mask):
     I = K.shape[0]
     return lib().orc_fluence(np.ascontiguousarray(W, np.float64), np.ascontiguousarray(K, np.float64),
                              np.ascontiguousarray(mask, np.int32), S, I, P)
+
+
+def model_radial(L, K):
+    """model_input = 3 (emc_helper.cpp:457-500) from patterns K [I][L*L]."""
+    K = np.ascontiguousarray(K, np.float32).reshape(-1, L * L)
+    out = np.zeros(L ** 3, np.float64)
+    lib().orc_model_radial(L, K, K.shape[0], out)
+    return out
+
+
+def model_orientations(L, K, mask, coords, q):
+    """model_input = 1 (emc_helper.cpp:501-516) with the orientations q [I][4] given."""
+    K = np.ascontiguousarray(K, np.float32).reshape(-1, L * L)
+    x, y, z = (np.ascontiguousarray(c, np.float32) for c in coords)
+    out = np.zeros(L ** 3, np.float64)
+    lib().orc_model_orientations(L, K, K.shape[0], np.ascontiguousarray(mask, np.int32), x, y, z,
+                                 np.ascontiguousarray(q, np.float32).reshape(-1, 4), out)
+    return out
+
+
+def normalize_patterns(K, mask):
+    """normalize_images (main.cpp:96-97, emc_helper.cpp:389-419)."""
+    K = np.ascontiguousarray(K, np.float32)
+    out = np.zeros(K.shape, np.float64)
+    lib().orc_normalize_patterns(K, K.shape[0], K.shape[1], np.ascontiguousarray(mask, np.int32), out)
+    return out
 
 
 def scaling(W, K, mask):

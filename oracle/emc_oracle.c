@@ -486,6 +486,85 @@ ORC_API void orc_scaling(const double *W, const double *K, const int32_t *mask, 
     }
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* Model initialisation (emc_helper.cpp:440-534) and pattern normalisation (:389-419).          */
+/* Pinned by: constant patterns give the constant inside L/2 and -1 outside; a brute-force      */
+/* numpy shell average; one pattern at the identity rotation lands shifted by (+1, +1) on plane */
+/* z = L/2 + 1 (the +0.5 of the host insert); normalised patterns sum to P over the mask (tests). */
+/* ------------------------------------------------------------------------------------------ */
+/* model_input = 3.  radavg[L/2] (read past the reference's array) is 0 here. */
+ORC_API void orc_model_radial(int L, const float *K, int I, double *model)
+{
+    const int H = L / 2;
+    double *sum = (double *)calloc((size_t)H + 1, sizeof(double));
+    double *cnt = (double *)calloc((size_t)H + 1, sizeof(double));
+    for (int i = 0; i < I; i++)
+        for (int x = 0; x < L; x++)
+            for (int y = 0; y < L; y++) {
+                const double dx = x - L / 2.0 + 0.5, dy = y - L / 2.0 + 0.5;
+                const double r = sqrt(dx * dx + dy * dy);
+                if (r < L / 2.0) {
+                    sum[(int)r] += K[(size_t)i * L * L + (size_t)y * L + x];
+                    cnt[(int)r] += 1.0;
+                }
+            }
+    for (int r = 0; r < H; r++) sum[r] = cnt[r] > 0.0 ? sum[r] / cnt[r] : 0.0;
+    sum[H] = 0.0;
+    for (int x = 0; x < L; x++)
+        for (int y = 0; y < L; y++)
+            for (int z = 0; z < L; z++) {
+                const double dx = x - L / 2.0 + 0.5, dy = y - L / 2.0 + 0.5, dz = z - L / 2.0 + 0.5;
+                const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+                const int r = (int)rad;
+                model[((size_t)z * L + y) * L + x] = rad < L / 2.0 ? sum[r] + (sum[r + 1] - sum[r]) * (rad - r) : -1.0;
+            }
+    free(sum);
+    free(cnt);
+}
+
+/* model_input = 1 with the orientations q [I][4] given: insert_slice (emc_helper.cpp:80-123) of every
+ * pattern with weight 1 at round(L/2 + 0.5 + R x) (R x rounded to float, as the reference stores it),
+ * then num / weight (0 where no weight). */
+ORC_API void orc_model_orientations(int L, const float *K, int I, const int32_t *mask, const float *X,
+                                    const float *Y, const float *Z, const float *q, double *model)
+{
+    const size_t N3 = (size_t)L * L * L, P = (size_t)L * L;
+    double *num = (double *)calloc(N3, sizeof(double));
+    double *wt = (double *)calloc(N3, sizeof(double));
+    for (int i = 0; i < I; i++) {
+        const double q0 = q[4 * i], q1 = q[4 * i + 1], q2 = q[4 * i + 2], q3 = q[4 * i + 3];
+        for (size_t p = 0; p < P; p++) {
+            if (mask[p] != 1) continue;
+            const float nx = (float)((q0 * q0 + q1 * q1 - q2 * q2 - q3 * q3) * X[p] + 2.0 * (q1 * q2 - q0 * q3) * Y[p] +
+                                     2.0 * (q1 * q3 + q0 * q2) * Z[p]);
+            const float ny = (float)(2.0 * (q1 * q2 + q0 * q3) * X[p] + (q0 * q0 - q1 * q1 + q2 * q2 - q3 * q3) * Y[p] +
+                                     2.0 * (q2 * q3 - q0 * q1) * Z[p]);
+            const float nz = (float)(2.0 * (q1 * q3 - q0 * q2) * X[p] + 2.0 * (q2 * q3 + q0 * q1) * Y[p] +
+                                     (q0 * q0 - q1 * q1 - q2 * q2 + q3 * q3) * Z[p]);
+            const long vx = lround(L / 2.0 + 0.5 + nx), vy = lround(L / 2.0 + 0.5 + ny), vz = lround(L / 2.0 + 0.5 + nz);
+            if (vx < 0 || vx >= L || vy < 0 || vy >= L || vz < 0 || vz >= L) continue;
+            const size_t o = ((size_t)vz * L + vy) * L + vx;
+            num[o] += K[(size_t)i * P + p];
+            wt[o] += 1.0;
+        }
+    }
+    for (size_t o = 0; o < N3; o++) model[o] = wt[o] > 0.0 ? num[o] / wt[o] : 0.0;
+    free(num);
+    free(wt);
+}
+
+/* normalize_images: K * P / sum_{mask == 1} K (unchanged when that sum is 0). */
+ORC_API void orc_normalize_patterns(const float *K, int I, int P, const int32_t *mask, double *out)
+{
+    for (int i = 0; i < I; i++) {
+        double s = 0.0;
+        for (int p = 0; p < P; p++)
+            if (mask[p] == 1) s += K[(size_t)i * P + p];
+        const double f = s != 0.0 ? (double)P / s : 1.0;
+        for (int p = 0; p < P; p++) out[(size_t)i * P + p] = K[(size_t)i * P + p] * f;
+    }
+}
+
 /* Global fluence for true Poisson: mean photons per pattern over mean slice sum (both over
  * unmasked pixels; slice sum over V_s).  No per-pattern scaling. */
 ORC_API double orc_fluence(const double *W, const double *K, const int32_t *mask, int S, int I,

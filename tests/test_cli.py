@@ -52,7 +52,7 @@ def test_cli_synthetic_run(tmp_path):
     w = np.fromfile(tmp_path / "output" / "weight_0001.bin", np.float32)
     assert m.size == 24 ** 3 and w.size == 24 ** 3
     assert abs(m[m > 0].mean() - 1.0) < 1e-4   # normalised to mean 1 over positive voxels
-    assert np.all((w > 0) | (m == -1) | (m < 0))
+    assert np.all(m >= 0) and np.all((m == 0) | (w > 0))  # write_model's mask (file_helper.cpp:50-65)
     assert all(l["search"] == "full" for l in lines)
 
 
@@ -86,3 +86,49 @@ def test_cli_reference_default_scaling_run(tmp_path):
     assert all(np.isfinite(l["total_logl"]) for l in lines)
     m = np.fromfile(tmp_path / "output" / "model_0001.bin", np.float32)
     assert abs(m[m > 0].mean() - 1.0) < 1e-4
+
+
+@pytest.mark.gpu
+def test_cli_reference_conf_choices(tmp_path):
+    """The rest of the reference's emc.conf choices: normalize_images = true (FP32 path), the radial
+    average initial model (model_input = 3) and the responsibility dump of isDebug."""
+    print("[cli] normalize_images, model_input = 3, isDebug", flush=True)
+    p = tmp_path / "emc.conf"
+    p.write_text(CONF + 'diff_type = "poisson";\nknown_intensity = false;\nnormalize_images = true;\n'
+                 "model_input = 3;\nisDebug = 4;\n")
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    assert [l["iteration"] for l in lines] == [0, 1] and all(np.isfinite(l["total_logl"]) for l in lines)
+    S = 420  # rotations_n = 2: 10 (n + 5 n^3)
+    for it in (0, 1):
+        R = np.fromfile(tmp_path / "debug" / f"respons_{it:04d}.data", np.float32)
+        assert R.size == S * 64
+        assert np.allclose(R.reshape(S, 64).sum(0), 1.0, atol=1e-4)  # normalised over rotations
+
+
+@pytest.mark.gpu
+def test_cli_model_from_file(tmp_path):
+    """model_input = 2 reads the initial model from model_file (raw float32, the driver's own output
+    format); model_input = 1 inserts the patterns at random orientations."""
+    print("[cli] model_input = 2 and 1", flush=True)
+    L = 24
+    rng = np.random.default_rng(0)
+    rng.uniform(0.5, 1.5, L ** 3).astype(np.float32).tofile(tmp_path / "init.bin")
+    for extra in (f'model_input = 2;\nmodel_file = "{tmp_path / "init.bin"}";\n', "model_input = 1;\n"):
+        p = tmp_path / "emc.conf"
+        p.write_text(CONF.replace("max_iterations = 2;", "max_iterations = 1;") + extra)
+        r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+        assert len(lines) == 1 and np.isfinite(lines[0]["total_logl"])
+
+
+def test_cli_rejects_bad_model_input(tmp_path):
+    p = tmp_path / "emc.conf"
+    p.write_text(CONF + "model_input = 7;\n")
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 2 and "model_input" in r.stderr
+    p.write_text(CONF + 'model_input = 2;\nmodel_file = "/nonexistent/model.bin";\n')
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 1 and "model_read_raw" in r.stderr
