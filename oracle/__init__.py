@@ -60,6 +60,8 @@ def lib():
         L.orc_model_orientations.restype = None
         L.orc_normalize_patterns.argtypes = [f32p, C.c_int, C.c_int, i32p, f64p]
         L.orc_normalize_patterns.restype = None
+        L.orc_scaling_reference.argtypes = [f32p, f32p, i32p, C.c_int, C.c_int, C.c_int, f64p]
+        L.orc_scaling_reference.restype = None
         L.orc_scaling.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int, f64p]
         L.orc_scaling.restype = None
         L.orc_fluence.argtypes = [f64p, f64p, i32p, C.c_int, C.c_int, C.c_int]
@@ -189,6 +191,19 @@ def scaling(W, K, mask):
     I = K.shape[0]
     phi = np.zeros((S, I), np.float64)
     lib().orc_scaling(W, K, np.ascontiguousarray(mask, np.int32), S, I, P, phi)
+    return phi
+
+
+def scaling_reference(W, K, mask):
+    """The reference's own computation of the same phi (512 threads, int-truncated sums of K W;
+    emc_cuda.cu:438-455), for parity studies only."""
+    W = np.ascontiguousarray(W, np.float32)
+    K = np.ascontiguousarray(K, np.float32)
+    S, P = W.shape
+    I = K.shape[0]
+    phi = np.zeros((S, I), np.float64)
+    with np.errstate(all="ignore"):
+        lib().orc_scaling_reference(W, K, np.ascontiguousarray(mask, np.int32), S, I, P, phi)
     return phi
 
 

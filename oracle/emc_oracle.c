@@ -486,6 +486,43 @@ ORC_API void orc_scaling(const double *W, const double *K, const int32_t *mask, 
     }
 }
 
+/* Reference-emulation mode of the same scaling (parity mode only; the library computes the exact */
+/* sums above).  calculate_scaling_absolute with 512 threads (TNUM, emc_cuda_help.h:8): thread t   */
+/* takes pixels t, t+512, ...; its sum of K^2 is a float, its sum of K W an int that truncates the */
+/* float (int + K W) on every add; inblock_reduce (emc_cuda.cu:52-60) then adds the 512 partials in */
+/* a halving tree (float and int); phi = float / int (0 gives inf or nan, as in the reference).    */
+/* Pinned by: equal to orc_scaling when every K W and every partial sum is an integer; below it  */
+/* (truncation) otherwise (tests).                                                                */
+ORC_API void orc_scaling_reference(const float *W, const float *K, const int32_t *mask, int S, int I, int P,
+                                   double *phi)
+{
+    enum { T = 512 };
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int s = 0; s < S; s++) {
+        float fs[T];
+        int iw[T];
+        const float *Ws = W + (size_t)s * P;
+        for (int i = 0; i < I; i++) {
+            const float *Ki = K + (size_t)i * P;
+            for (int t = 0; t < T; t++) {
+                fs[t] = 0.0f;
+                iw[t] = 0;
+                for (int p = t; p < P; p += T)
+                    if (mask[p] > 0 && (double)Ws[p] > 1e-10 && (double)Ki[p] > 1e-10) {
+                        fs[t] = fs[t] + Ki[p] * Ki[p];
+                        iw[t] = (int)((float)iw[t] + Ki[p] * Ws[p]);
+                    }
+            }
+            for (int h = T / 2; h > 0; h >>= 1)
+                for (int t = 0; t < h; t++) {
+                    fs[t] += fs[t + h];
+                    iw[t] += iw[t + h];
+                }
+            phi[(size_t)s * I + i] = (double)(fs[0] / (float)iw[0]);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Model initialisation (emc_helper.cpp:440-534) and pattern normalisation (:389-419).          */
 /* Pinned by: constant patterns give the constant inside L/2 and -1 outside; a brute-force      */

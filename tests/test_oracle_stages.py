@@ -170,6 +170,22 @@ def test_scaling_closed_forms_and_brute_force(orc):
             assert abs(phr[s, i] - (num / den if num > 0 and den > 0 else 1.0)) < 1e-12
 
 
+def test_scaling_reference_mode_truncates(orc):
+    """The reference's int accumulation of sum K W: exact when every product and partial sum is an
+    integer, below the real sum (phi above) otherwise."""
+    rng = np.random.default_rng(19)
+    S, P = 3, 1500
+    mask = np.ones(P, np.int32)
+    Wi = rng.integers(1, 4, size=(S, P)).astype(np.float64)
+    K = rng.poisson(1.5, size=(4, P)).astype(np.float64)
+    assert np.allclose(orc.scaling_reference(Wi, K, mask), orc.scaling(Wi, K, mask), rtol=1e-6)
+    Wf = rng.uniform(0.2, 2.0, size=(S, P))
+    ref, exact = orc.scaling_reference(Wf, K, mask), orc.scaling(Wf, K, mask)
+    assert np.all(ref > exact * 1.05)  # truncation drops a large share of sum K W at these counts
+    zero = orc.scaling_reference(Wf, np.zeros((1, P)), mask)
+    assert not np.isfinite(zero).any()  # 0 / 0, as the reference
+
+
 @pytest.mark.parametrize("kind", ["poisson", "absolute"])
 def test_scaled_copy_of_a_slice_is_a_perfect_match(orc, kind):
     """With phi from the scaling step, a pattern that is c times slice s has K/phi = W: D = 0."""
