@@ -331,16 +331,35 @@ __global__ __launch_bounds__(256) void k_cell_targets(const bf16_t *__restrict__
                                                       const int32_t *__restrict__ rbeg, float *__restrict__ Tt,
                                                       float *__restrict__ Mx)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
-    if (i >= I) return;
-    float acc = 0.0f, mx = -INFINITY;
-    for (int g = rbeg[c]; g < rbeg[c + 1]; ++g) {
-        const size_t o = (size_t)cell_rot[g] * ld + i;
-        acc += bf2f(Ph[o]) + bf2f(Pl[o]);
-        if (LOGITS) mx = fmaxf(mx, Lg[o]);
+    // eight patterns per thread: 16-byte loads of hi and lo, two of the logits (i0 < I implies
+    // i0 + 8 <= ld); each pattern's sum runs over the cell's rotations in order
+    const int i0 = (blockIdx.x * 256 + threadIdx.x) * 8, c = blockIdx.y;
+    if (i0 >= I) return;
+    float acc[8], mx[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        acc[k] = 0.0f;
+        mx[k] = -INFINITY;
     }
-    Tt[(size_t)c * ld + i] = acc;
-    if (LOGITS) Mx[(size_t)c * ld + i] = mx;
+    for (int g = rbeg[c]; g < rbeg[c + 1]; ++g) {
+        const size_t o = (size_t)cell_rot[g] * ld + i0;
+        const u32x4 h = *reinterpret_cast<const u32x4 *>(Ph + o), l = *reinterpret_cast<const u32x4 *>(Pl + o);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            acc[k] += bf2f((bf16_t)(h[k >> 1] >> (16 * (k & 1)))) + bf2f((bf16_t)(l[k >> 1] >> (16 * (k & 1))));
+        if (LOGITS) {
+            const float4 a = *reinterpret_cast<const float4 *>(Lg + o), b = *reinterpret_cast<const float4 *>(Lg + o + 4);
+            const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+            for (int k = 0; k < 8; ++k) mx[k] = fmaxf(mx[k], v[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (i0 + k < I) {
+            Tt[(size_t)c * ld + i0 + k] = acc[k];
+            if (LOGITS) Mx[(size_t)c * ld + i0 + k] = mx[k];
+        }
 }
 
 // Responsibility mass inside the cells a proposal iteration would search for each pattern: the
@@ -579,7 +598,7 @@ hipError_t mlp_union_mass(const emc_ctx *c, hipStream_t st)
 hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, float *Mx, hipStream_t st)
 {
     const Proposal *pr = c->prop;
-    const dim3 grid((c->I + 255) / 256, pr->C);
+    const dim3 grid((c->I + 2047) / 2048, pr->C);  // eight patterns per thread
     if (Mx)
         hipLaunchKernelGGL(k_cell_targets<true>, grid, dim3(256), 0, st, c->Ph, c->Pl, c->R, (int64_t)c->Ip, c->I,
                            pr->cell_rot, pr->rbeg, Tt, Mx);
