@@ -316,6 +316,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
+    ap.add_argument("--mstep", default="auto", choices=["auto", "dense", "sparse"],
+                    help="M-step form of the headline run (auto: chosen on the device per iteration)")
     ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
     ap.add_argument("--accuracy", action="store_true", help="precision against the FP64 oracle (slow)")
     ap.add_argument("--no-proposal", action="store_true",
@@ -346,7 +348,7 @@ def main():
     log(f"[bench] workload {args.workload}: S={S} I={I} L={L} P_act={Pa} max count {d['K'].max()} "
         f"(generated in {time.perf_counter() - t:.1f}s)")
 
-    ms_mfma, st_mfma, last = run_engine(d, "bf16x2", args.steps, args.warmup, args.insert)
+    ms_mfma, st_mfma, last = run_engine(d, "bf16x2", args.steps, args.warmup, args.insert, mstep=args.mstep)
     fp32 = None
     if not args.no_fp32:
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
@@ -410,7 +412,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "model_side": L, "rotations": S, "patterns": I,
                    "active_pixels": Pa, "photons": d["cfg"]["photons"], "precision": "bf16x2",
-                   "insert": args.insert, "interp": "trilinear", "likelihood": "true_poisson",
+                   "insert": args.insert, "mstep": args.mstep, "interp": "trilinear", "likelihood": "true_poisson",
                    "blur_sigma": d["cfg"]["blur"], "parallelism": "single_gpu"},
         "fp32": fp32,
         "speedup_vs_fp32": (fp32["ms_per_step"] / ms_mfma) if fp32 else None,

@@ -14,11 +14,15 @@
 // raw float32 [rotations][patterns] (file_helper.cpp:14-27, main.cpp:394-416).
 //
 // Data: image_prefix = "synthetic" generates the workload from the synth_* keys (Gaussian-atom
-// object, flat or Ewald detector, annular mask, Poisson patterns).  normalize_images scales each
+// object, flat or Ewald detector, annular mask, Poisson patterns); otherwise the patterns are the
+// HDF5 files image_prefix%04d.h5 and the mask mask_file, binned by read_stride (needs libemc_h5.so).  normalize_images scales each
 // pattern by P / its masked sum (main.cpp:96-97).  Initial model by model_input
 // (emc_helper.cpp:440-534): 0 uniform random, 1 the patterns inserted at random orientations,
-// 2 read from model_file (raw float32 [L^3]), 3 the patterns' radial average.
+// 2 read from model_file (raw float32 [L^3], or HDF5 "/real" for a .h5 name), 3 the patterns'
+// radial average.
+#include <dlfcn.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <chrono>
 #include <cmath>
@@ -37,6 +41,34 @@ int die(const char *what, emc_status st, const emc_ctx *ctx)
     return 1;
 }
 
+// The optional HDF5 formats (libemc_h5.so next to libemc.so), loaded when present.
+struct H5Api {
+    bool ok = false;
+    emc_status (*read_pattern)(const char *, int32_t, int32_t, float *, int32_t *) = nullptr;
+    emc_status (*read_mask)(const char *, int32_t, int32_t, int32_t *) = nullptr;
+    emc_status (*write_model)(const char *, int32_t, const float *, const float *) = nullptr;
+    emc_status (*read_model)(const char *, int32_t, float *) = nullptr;
+};
+
+H5Api load_h5()
+{
+    H5Api a;
+    char exe[4096];
+    const ssize_t n = readlink("/proc/self/exe", exe, sizeof exe - 1);
+    if (n <= 0) return a;
+    exe[n] = '\0';
+    std::string dir(exe);
+    dir = dir.substr(0, dir.find_last_of('/'));
+    void *h = dlopen((dir + "/../libemc_h5.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+    if (!h) return a;
+    a.read_pattern = reinterpret_cast<decltype(a.read_pattern)>(dlsym(h, "emc_h5_read_pattern"));
+    a.read_mask = reinterpret_cast<decltype(a.read_mask)>(dlsym(h, "emc_h5_read_mask"));
+    a.write_model = reinterpret_cast<decltype(a.write_model)>(dlsym(h, "emc_h5_write_model"));
+    a.read_model = reinterpret_cast<decltype(a.read_model)>(dlsym(h, "emc_h5_read_model"));
+    a.ok = a.read_pattern && a.read_mask && a.write_model && a.read_model;
+    return a;
+}
+
 bool write_raw(const char *path, const float *data, size_t n)
 {
     FILE *f = std::fopen(path, "wb");
@@ -53,9 +85,11 @@ int main(int argc, char **argv)
     emc_config cfg;
     emc_status st = emc_config_read(path, &cfg);
     if (st != EMC_OK) return die("reading the configuration", st, nullptr);
-    if (std::strcmp(cfg.image_prefix, "synthetic") != 0 && cfg.image_prefix[0] != '\0') {
-        std::fprintf(stderr, "emc: reading image files (%s) is not part of this build; set image_prefix = "
-                             "\"synthetic\"\n", cfg.image_prefix);
+    const bool synthetic = std::strcmp(cfg.image_prefix, "synthetic") == 0 || cfg.image_prefix[0] == '\0';
+    const H5Api h5 = load_h5();
+    if (!synthetic && !h5.ok) {
+        std::fprintf(stderr, "emc: reading image files (%s) needs libemc_h5.so (built where HDF5 is present); "
+                             "set image_prefix = \"synthetic\" otherwise\n", cfg.image_prefix);
         return 2;
     }
     const int L = cfg.model_side, P = L * L, I = cfg.N_images;
@@ -69,19 +103,39 @@ int main(int argc, char **argv)
                                               (float)cfg.wavelength, x.data(), y.data(), z.data());
     if (st != EMC_OK) return die("coordinates", st, nullptr);
     std::vector<int32_t> mask(P);
-    if (emc_mask_annulus(L, cfg.beamstop, cfg.outer_radius, mask.data()) <= 0) return die("mask", EMC_E_INVALID, nullptr);
-
     const size_t N3 = (size_t)L * L * L;
-    std::vector<float> truth(N3), model(N3), weight(N3);
-    std::vector<int32_t> K((size_t)I * P);
+    std::vector<float> model(N3), weight(N3);
+    std::vector<int32_t> K;  // integer counts (synthetic data)
+    std::vector<float> Kf;   // float patterns (image files, or normalised)
     const uint64_t seed = cfg.synth_seed;
-    if ((st = emc_synth_model(L, cfg.synth_atoms, cfg.synth_radius, 1000 + seed, truth.data())) != EMC_OK)
-        return die("synthetic model", st, nullptr);
-    if ((st = emc_synth_patterns(truth.data(), L, x.data(), y.data(), z.data(), mask.data(), I, cfg.synth_photons,
-                                 2000 + seed, K.data(), nullptr)) != EMC_OK)
-        return die("synthetic patterns", st, nullptr);
-    std::vector<float> Kf;
-    if (cfg.normalize_images) {
+    if (synthetic) {
+        if (emc_mask_annulus(L, cfg.beamstop, cfg.outer_radius, mask.data()) <= 0)
+            return die("mask", EMC_E_INVALID, nullptr);
+        std::vector<float> truth(N3);
+        K.resize((size_t)I * P);
+        if ((st = emc_synth_model(L, cfg.synth_atoms, cfg.synth_radius, 1000 + seed, truth.data())) != EMC_OK)
+            return die("synthetic model", st, nullptr);
+        if ((st = emc_synth_patterns(truth.data(), L, x.data(), y.data(), z.data(), mask.data(), I, cfg.synth_photons,
+                                     2000 + seed, K.data(), nullptr)) != EMC_OK)
+            return die("synthetic patterns", st, nullptr);
+    } else {
+        // image files "%s%04d.h5" binned read_stride x read_stride, the global mask from mask_file
+        // (main.cpp:93-95); the per-image masks are not used by the iteration (as the reference's kernels)
+        if ((st = h5.read_mask(cfg.mask_file, L, cfg.read_stride, mask.data())) != EMC_OK)
+            return die("reading mask_file", st, nullptr);
+        Kf.resize((size_t)I * P);
+        std::vector<int32_t> own(P);
+        for (int i = 0; i < I; ++i) {
+            char name[sizeof cfg.image_prefix + 16];
+            std::snprintf(name, sizeof name, "%s%04d.h5", cfg.image_prefix, i);
+            if ((st = h5.read_pattern(name, L, cfg.read_stride, Kf.data() + (size_t)i * P, own.data())) != EMC_OK)
+                return die("reading the images", st, nullptr);
+        }
+    }
+    if (!Kf.empty() && cfg.normalize_images) {
+        if ((st = emc_normalize_patterns(Kf.data(), I, P, mask.data(), Kf.data())) != EMC_OK)
+            return die("normalize_images", st, nullptr);
+    } else if (cfg.normalize_images) {
         Kf.assign(K.begin(), K.end());
         if ((st = emc_normalize_patterns(Kf.data(), I, P, mask.data(), Kf.data())) != EMC_OK)
             return die("normalize_images", st, nullptr);
@@ -91,16 +145,25 @@ int main(int argc, char **argv)
         st = emc_synth_uniform_model(L, 3000 + seed, model.data());
         break;
     case 1:
-    case 3:
-        if (Kf.empty()) Kf.assign(K.begin(), K.end());
+    case 3: {
+        const bool tmp = Kf.empty();
+        if (tmp) Kf.assign(K.begin(), K.end());
         st = cfg.model_input == 1 ? emc_model_init_orientations(L, Kf.data(), I, mask.data(), x.data(), y.data(),
                                                                 z.data(), 3000 + seed, nullptr, model.data())
                                   : emc_model_init_radial(L, Kf.data(), I, model.data());
-        if (!cfg.normalize_images) std::vector<float>().swap(Kf);
+        if (tmp) std::vector<float>().swap(Kf);
         break;
-    case 2:
-        st = emc_model_read_raw(cfg.model_file, L, model.data());
+    }
+    case 2: {
+        const size_t n = std::strlen(cfg.model_file);
+        const bool is_h5 = n > 3 && std::strcmp(cfg.model_file + n - 3, ".h5") == 0;
+        if (is_h5 && !h5.ok) {
+            std::fprintf(stderr, "emc: an .h5 model_file needs libemc_h5.so\n");
+            return 2;
+        }
+        st = is_h5 ? h5.read_model(cfg.model_file, L, model.data()) : emc_model_read_raw(cfg.model_file, L, model.data());
         break;
+    }
     default:
         std::fprintf(stderr, "emc: model_input must be 0, 1, 2 or 3\n");
         return 2;
@@ -196,6 +259,10 @@ int main(int argc, char **argv)
                 bool ok = write_raw(name, model.data(), N3);
                 std::snprintf(name, sizeof name, "output/weight_%04d.bin", it);
                 ok = ok && write_raw(name, weight.data(), N3);
+                if (h5.ok) {  // write_model's HDF5 file beside the raw volumes
+                    std::snprintf(name, sizeof name, "output/model_%04d.h5", it);
+                    ok = ok && h5.write_model(name, L, model.data(), weight.data()) == EMC_OK;
+                }
                 if (!ok) {
                     std::fprintf(stderr, "emc: cannot write output files\n");
                     rc = 1;

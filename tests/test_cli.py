@@ -7,6 +7,8 @@ import subprocess
 import numpy as np
 import pytest
 
+from emc_amd import h5
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "emc_amd", "bin", "emc")
 
@@ -29,11 +31,14 @@ def test_cli_rejects_bad_config(tmp_path):
     assert r.returncode != 0 and "no_such_key" in r.stderr
 
 
-def test_cli_rejects_file_images(tmp_path):
+def test_cli_missing_image_files(tmp_path):
     p = tmp_path / "emc.conf"
-    p.write_text('image_prefix = "/data/img_";\n')
+    p.write_text('image_prefix = "/nonexistent/img_";\nmask_file = "/nonexistent/mask.h5";\n')
     r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path)
-    assert r.returncode == 2 and "synthetic" in r.stderr
+    if h5.available():
+        assert r.returncode == 1 and "mask_file" in r.stderr
+    else:
+        assert r.returncode == 2 and "libemc_h5" in r.stderr
 
 
 @pytest.mark.gpu
@@ -132,3 +137,26 @@ def test_cli_rejects_bad_model_input(tmp_path):
     p.write_text(CONF + 'model_input = 2;\nmodel_file = "/nonexistent/model.bin";\n')
     r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode == 1 and "model_read_raw" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not h5.available(), reason="libemc_h5.so not built (no HDF5 library)")
+def test_cli_reads_hdf5_images(tmp_path):
+    """image_prefix%04d.h5 patterns (48 x 48 detector, read_stride 2 onto 24 x 24) and an HDF5
+    mask_file; the model is also written as output/model_%04d.h5."""
+    print("[cli] HDF5 images", flush=True)
+    rng = np.random.default_rng(3)
+    for i in range(32):
+        h5.write_image(tmp_path / f"img_{i:04d}.h5", rng.poisson(2.0, size=(48, 48)).astype(np.float32))
+    h5.write_image(tmp_path / "mask.h5", np.ones((48, 48), np.float32))
+    conf = CONF.replace('image_prefix = "synthetic";', f'image_prefix = "{tmp_path}/img_";') \
+               .replace("N_images = 64;", "N_images = 32;").replace("max_iterations = 2;", "max_iterations = 1;")
+    p = tmp_path / "emc.conf"
+    p.write_text(conf + f'mask_file = "{tmp_path}/mask.h5";\nread_stride = 2;\nmodel_input = 3;\n')
+    r = subprocess.run([BIN, str(p)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(l) for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(lines) == 1 and np.isfinite(lines[0]["total_logl"])
+    m = h5.read_model(tmp_path / "output" / "model_0000.h5", 24)
+    raw = np.fromfile(tmp_path / "output" / "model_0000.bin", np.float32)
+    assert np.array_equal(m, raw)  # the same write_model mask on both files
