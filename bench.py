@@ -68,6 +68,37 @@ def run_engine(d, prec, steps, warmup, insert="binned", mfma_tile=0, **extra):
     return ms, stage, stats[-1]
 
 
+def stage_rooflines(st, S, I, Pa, L, ms_iter):
+    """Each stage's algorithmic work, its bound and the achieved fraction of that bound, and the
+    iteration's roofline fraction: sum over stages of the bound's minimum time / the iteration time.
+    Bytes and FLOPs are algorithmic (what the stage must move or compute once), from the stage times
+    the library measures with HIP events on its stream."""
+    N3 = L ** 3
+    hbm = PEAK["hbm_gbs"] * 1e9
+    mfma = PEAK["bf16_tflops"] * 1e12
+    t_ins = max(st["mstep"] - st["mstep_gemm"], 1e-9)
+    work = {
+        # name: (bound, units, per-iteration amount, stage time in ms)
+        "slice": ("hbm", "B", S * Pa * 4 + N3 * 4, st["slice"] + (st["logl"] - st["logl_gemm"])),
+        "logl_contraction": ("mfma", "FLOP", 2.0 * I * S * Pa, st["logl_gemm"]),
+        "softmax": ("hbm", "B", S * I * 4 + S * I * 4, st["softmax"]),
+        "mstep_sparse_scan": ("hbm", "B", S * I * 2, st["mstep_gemm"]),
+        "insertion": ("hbm", "B", S * Pa * 4 + 2 * N3 * 4, t_ins),
+        "compress_blur_normalise": ("hbm", "B", N3 * 4 * (3 + 2 * 3 + 2), st["compress"]),
+    }
+    out, t_min = {}, 0.0
+    for k, (bound, unit, amount, ms) in work.items():
+        peak = hbm if bound == "hbm" else mfma
+        t_bound = amount / peak
+        t_min += t_bound
+        rate = amount / (ms * 1e-3)
+        out[k] = {"bound": bound, "algorithmic": amount, "unit": unit, "ms": ms,
+                  "achieved": rate / (1e9 if bound == "hbm" else 1e12),
+                  "achieved_unit": "GB/s" if bound == "hbm" else "TFLOP/s", "frac": t_bound / (ms * 1e-3)}
+    out["iteration"] = {"roofline_ms": t_min * 1e3, "ms": ms_iter, "frac": t_min * 1e3 / ms_iter}
+    return out
+
+
 def cpu_baseline(d, frac_rot: int):
     """FP64 oracle (OpenMP) on every pattern and a contiguous 1/frac_rot share of the rotations;
     the per-iteration time is extrapolated linearly in the rotation count."""
@@ -377,6 +408,9 @@ def main():
             "algorithmic": f"2*I*S*P_act = {flops:.3e} FLOP per launch (one launch per iteration)",
             "mstep_achieved": flops / (st_mfma["mstep_gemm"] * 1e-3) / 1e12}
 
+    stages = stage_rooflines(st_mfma, S, I, Pa, L, ms_mfma)
+    log(f"[bench] stage rooflines {json.dumps(stages)}")
+
     acc = accuracy(d, 400, 4000) if args.accuracy else None
     if acc:
         log(f"[bench] accuracy {json.dumps(acc)}")
@@ -419,6 +453,7 @@ def main():
         "stage_ms": st_mfma,
         "total_logl": last["total_logl"],
         "roofline": roof,
+        "stage_rooflines": stages,
         "cpu_baseline": cpu,
         "variants": variants or None,
         "accuracy": acc,
