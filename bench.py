@@ -385,9 +385,13 @@ def main():
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
         fp32 = {"value": 1000.0 / ms32, "ms_per_step": ms32, "stage_ms": st32,
                 "logl_gemm_tflops": 2.0 * I * S * Pa / (st32["logl_gemm"] * 1e-3) / 1e12}
-        # the reference's own emc.conf choices: chi^2 ("poisson") with known_intensity = false
+        # the reference's own emc.conf choices: chi^2 ("poisson") with known_intensity = false, on
+        # the plain FP32 path and on the matrix cores (expanded contractions)
         mss, sts, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic", diff="poisson", scaling=1)
         fp32["chi2_scaled"] = {"value": 1000.0 / mss, "ms_per_step": mss, "stage_ms": sts}
+        msm, stm, _ = run_engine(d, "bf16x2", args.steps, 1, args.insert, diff="poisson", scaling=1)
+        fp32["chi2_scaled_mfma"] = {"value": 1000.0 / msm, "ms_per_step": msm, "stage_ms": stm,
+                                    "speedup_vs_fp32": mss / msm}
 
     variants = {}
     if args.variants:

@@ -201,6 +201,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
+    dfree(c->Cx); dfree(c->K2h); dfree(c->K2l); dfree(c->Aw); dfree(c->G2); dfree(c->G3); dfree(c->G4);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
     dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
@@ -226,10 +227,8 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         opt.mstep < 0 || opt.mstep > 2 || opt.scaling < 0 || opt.scaling > 1 ||
         opt.mfma_tile > 1)
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
-    if (opt.precision != EMC_PREC_FP32 && opt.diff != EMC_DIFF_TRUE_POISSON)
-        return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: MFMA precision modes need diff = TRUE_POISSON");
-    if (opt.scaling && (opt.precision != EMC_PREC_FP32 || opt.diff == EMC_DIFF_TRUE_POISSON))
-        return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: intensity scaling needs precision FP32 and a chi^2 diff");
+    if (opt.scaling && opt.diff == EMC_DIFF_TRUE_POISSON)
+        return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: intensity scaling needs a chi^2 diff");
     if (opt.sigma_half_life == 0 && opt.diff != EMC_DIFF_TRUE_POISSON)
         return fail(nullptr, EMC_E_INVALID, "emc_create: sigma_half_life must be non-zero");
 
@@ -245,6 +244,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     c->chunkp = (int)round_up(c->chunk, 256);
     c->mfma = opt.precision != EMC_PREC_FP32;
     c->la_terms = opt.precision == EMC_PREC_BF16X3 ? 3 : 2;
+    c->chi2_mfma = c->mfma && opt.diff != EMC_DIFF_TRUE_POISSON;
     const int nch = (c->S + c->chunk - 1) / c->chunk;
     c->Sp = (int)round_up(std::max<int64_t>((int64_t)(nch - 1) * c->chunk + c->chunkp, c->S), kPadS);
     c->Ip = (int)round_up(c->I, kPadI);
@@ -282,6 +282,22 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK) A(dalloc(&c->Pl, Sp * Ip), "Pl");
         if (st == EMC_OK) A(dalloc(&c->colsum, 2 * (int64_t)PpMax), "colsum");
         if (st == EMC_OK) A(dalloc(&c->lw32, Sp * PpMax), "lw32");
+        if (c->chi2_mfma) {
+            const int64_t nops = (opt.scaling ? 4 : 2) * (int64_t)c->la_terms;
+            if (st == EMC_OK) A(dalloc(&c->A0, cp * PpMax), "A0");
+            if (st == EMC_OK) A(dalloc(&c->A1, cp * PpMax), "A1");
+            if (st == EMC_OK) A(dalloc(&c->Cx, nops * cp * PpMax), "Cx");
+            if (st == EMC_OK) A(dalloc(&c->K2h, Ip * PpMax), "K2h");
+            if (st == EMC_OK) A(dalloc(&c->K2l, Ip * PpMax), "K2l");
+            if (st == EMC_OK) A(dalloc(&c->Aw, cp), "Aw");
+            if (st == EMC_OK) A(dalloc(&c->G2, cp * Ip), "G2");
+            if (opt.scaling) {
+                if (st == EMC_OK) A(dalloc(&c->G3, cp * Ip), "G3");
+                if (st == EMC_OK) A(dalloc(&c->G4, cp * Ip), "G4");
+                if (st == EMC_OK) A(dalloc(&c->iphi, Sp * Ip), "iphi");
+                if (st == EMC_OK) A(launch_fill_f32(c->iphi, 1.0f, Sp * Ip, nullptr), "iphi");
+            }
+        }
     } else {
         if (st == EMC_OK) A(dalloc(&c->K32, Ip * PpMax), "K32");
         if (st == EMC_OK) A(dalloc(&c->KT32, Ip * PpMax), "KT32");
@@ -417,6 +433,8 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
     if (c->mfma) {
         if (!integral) return fail(c, EMC_E_UNSUPPORTED, "set_patterns: MFMA precision modes need integer counts");
         if (maxc > 65535.0) return fail(c, EMC_E_UNSUPPORTED, "set_patterns: counts above 65535 need precision = FP32");
+        if (c->chi2_mfma && maxc > 255.0)
+            return fail(c, EMC_E_UNSUPPORTED, "set_patterns: the chi^2 kinds on MFMA need counts <= 255 (K^2 exact in two BF16 terms)");
         c->k_split = maxc > 256.0;
         std::vector<bf16_t> hi((size_t)Ip * Pp, 0), lo, hT((size_t)Ip * Pp, 0), lT;
         if (c->k_split) {
@@ -461,6 +479,20 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
         }
         HIP_TRY(c, hipMemcpy(c->K16, hi.data(), sizeof(bf16_t) * hi.size(), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->KT16, hT.data(), sizeof(bf16_t) * hT.size(), hipMemcpyHostToDevice));
+        if (c->chi2_mfma) {
+            // K^2 < 2^16 as an exact BF16 pair: hi = bf16(K^2), lo = K^2 - hi (< 256, exact)
+            std::vector<bf16_t> h2((size_t)Ip * Pp, 0), l2((size_t)Ip * Pp, 0);
+            for (int i = 0; i < I; ++i)
+                for (int j = 0; j < Pa; ++j) {
+                    const float k = Kf[(size_t)i * P + pix[j]], k2 = k * k;
+                    const bf16_t h = f2bf(k2);
+                    h2[(size_t)i * Pp + j] = h;
+                    l2[(size_t)i * Pp + j] = f2bf(k2 - bf2f(h));
+                }
+            c->k2_split = maxc > 16.0;
+            HIP_TRY(c, hipMemcpy(c->K2h, h2.data(), sizeof(bf16_t) * h2.size(), hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->K2l, l2.data(), sizeof(bf16_t) * l2.size(), hipMemcpyHostToDevice));
+        }
         if (c->k_split) {
             HIP_TRY(c, hipMemcpy(c->K16lo, lo.data(), sizeof(bf16_t) * lo.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMemcpy(c->KT16lo, lT.data(), sizeof(bf16_t) * lT.size(), hipMemcpyHostToDevice));
@@ -594,6 +626,12 @@ extern "C" emc_status emc_get_resp(emc_ctx *c, float *resp)
     HIP_TRY(c, hipMemcpy2D(l.data(), sizeof(bf16_t) * c->I, c->Pl, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * c->I,
                            c->S, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < h.size(); ++k) resp[k] = bf2f(h[k]) + bf2f(l[k]);
+    if (c->opt.scaling) {  // the pair holds the M-step's P / phi: undo the factor
+        std::vector<float> ip((size_t)c->S * c->I);
+        HIP_TRY(c, hipMemcpy2D(ip.data(), sizeof(float) * c->I, c->iphi, sizeof(float) * c->Ip, sizeof(float) * c->I,
+                               c->S, hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < ip.size(); ++k) resp[k] /= ip[k];
+    }
     return EMC_OK;
 }
 
@@ -629,7 +667,7 @@ static emc_status slice_prepare(emc_ctx *c, float sigma)
     ev_start(c, EMC_STAGE_SLICE);
     HIP_TRY(c, launch_slice_stats(c, c->mfma, st));
     HIP_TRY(c, launch_bias(c, sigma, st));
-    HIP_TRY(c, launch_pattern_const(c, c->mfma, st));
+    HIP_TRY(c, launch_pattern_const(c, c->mfma && !c->chi2_mfma, st));
     ev_stop(c, EMC_STAGE_SLICE);
     return EMC_OK;
 }
@@ -642,7 +680,12 @@ static emc_status logl_enqueue(emc_ctx *c, float sigma)
     for (int s0 = 0; s0 < c->S; s0 += c->chunk) {
         const int cs = std::min(c->chunk, c->S - s0);
         const int k = s0 / c->chunk;
-        if (c->mfma) {
+        if (c->chi2_mfma) {
+            HIP_TRY(c, launch_slice_f32(c, s0, cs, st));
+            gev_rec(c, 0, k, 0);
+            HIP_TRY(c, launch_logl_chi2_mfma(c, s0, cs, st));
+            gev_rec(c, 0, k, 1);
+        } else if (c->mfma) {
             HIP_TRY(c, launch_slice_bf16(c, s0, cs, st));
             gev_rec(c, 0, k, 0);
             HIP_TRY(c, launch_logl_mfma(c, s0, cs, st));
@@ -728,10 +771,17 @@ extern "C" emc_status emc_set_resp(emc_ctx *c, const float *resp)
         c->has_logl = false;
     } else {
         std::vector<bf16_t> h((size_t)S * I), l((size_t)S * I);
+        std::vector<float> ip;
+        if (c->opt.scaling) {  // the pair holds P / phi (phi of the last likelihood pass, 1 before any)
+            ip.resize((size_t)S * I);
+            HIP_TRY(c, hipMemcpy2D(ip.data(), sizeof(float) * I, c->iphi, sizeof(float) * c->Ip, sizeof(float) * I, S,
+                                   hipMemcpyDeviceToHost));
+        }
         double nnz = 0.0;  // read by the M-step's choice of form, as after an E-step
         for (size_t k = 0; k < h.size(); ++k) {
-            h[k] = f2bf(resp[k]);
-            l[k] = f2bf(resp[k] - bf2f(h[k]));
+            const float q = ip.empty() ? resp[k] : resp[k] * ip[k];
+            h[k] = f2bf(q);
+            l[k] = f2bf(q - bf2f(h[k]));
             nnz += (h[k] | l[k]) != 0;
         }
         HIP_TRY(c, hipMemcpy(c->scal + SC_NNZ, &nnz, sizeof(double), hipMemcpyHostToDevice));

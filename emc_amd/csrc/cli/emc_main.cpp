@@ -24,6 +24,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -175,12 +176,15 @@ int main(int argc, char **argv)
     emc_default_options(&opt);
     opt.diff = cfg.diff;
     opt.interp = cfg.interp;
-    // the MFMA modes need the true Poisson likelihood and integer counts (exact in BF16); the chi^2
-    // kinds and normalised patterns run on the FP32 path
-    opt.precision = cfg.diff == EMC_DIFF_TRUE_POISSON && Kf.empty() ? cfg.precision : EMC_PREC_FP32;
+    // the MFMA modes need integer counts (exact in BF16), and at most 255 with a chi^2 diff_type
+    // (the squared counts are an exact BF16 pair); otherwise the FP32 path
+    int32_t kmax = 0;
+    for (int32_t k : K) kmax = std::max(kmax, k);
+    const bool mfma_ok = Kf.empty() && (cfg.diff == EMC_DIFF_TRUE_POISSON || kmax <= 255);
+    opt.precision = mfma_ok ? cfg.precision : EMC_PREC_FP32;
     if (opt.precision != cfg.precision)
         std::fprintf(stderr, "emc: %s runs on the FP32 path\n",
-                     cfg.diff != EMC_DIFF_TRUE_POISSON ? "a chi^2 diff_type" : "normalize_images");
+                     Kf.empty() ? "a chi^2 diff_type with counts above 255" : "non-integer patterns");
     opt.insert = cfg.insert;
     opt.sigma_start = cfg.sigma_start;
     opt.sigma_final = cfg.sigma_final;

@@ -119,6 +119,7 @@ struct emc_ctx {
     bool mfma = false;               // precision is a BF16 MFMA mode
     int la_terms = 2;                // BF16 terms of log W (2 or 3)
     bool k_split = false;            // counts need a hi+lo BF16 pair (max count > 256)
+    bool chi2_mfma = false;          // a chi^2 diff on the MFMA path (expanded contractions)
     double sumK_total = 0.0;         // sum of all counts over active pixels (FP64)
     int32_t max_count = 0;
     std::string err;
@@ -162,6 +163,14 @@ struct emc_ctx {
     float *K2 = nullptr;                  // [Ip][Pp] K^2
     float *iphi = nullptr;                // [Sp][Ip] (1 until the first likelihood pass)
     float *Pphi = nullptr;                // [Sp][Ip]
+    // chi^2 kinds on the MFMA path: the BF16-split operands of the four contractions
+    // (a W, a, 1[W > 1e-10], W 1[W > 1e-10]; la_terms terms each), counts squared as an exact
+    // hi + lo BF16 pair, per-rotation sum a W^2 and the products' FP32 buffers
+    emc::bf16_t *Cx = nullptr;                 // [4][la_terms][chunkp][Pp]
+    emc::bf16_t *K2h = nullptr, *K2l = nullptr; // [Ip][Pp]
+    bool k2_split = false;                     // some count above 16: K^2 needs the lo term
+    double *Aw = nullptr;                      // [chunkp]
+    float *G2 = nullptr, *G3 = nullptr, *G4 = nullptr; // [chunkp][Ip]
     emc::bf16_t *Lh = nullptr, *Lm = nullptr, *Ll = nullptr;       // [chunkp][Pp] BF16 log W terms
     float *lw32 = nullptr;    // [Sp][Pp] log W from the statistics pass (-inf where W <= 0), MFMA path
     // responsibilities
@@ -220,6 +229,16 @@ hipError_t launch_pscale_f32(const emc_ctx *c, hipStream_t st);
 hipError_t launch_square_counts(const emc_ctx *c, hipStream_t st);
 // MFMA contractions.
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st);
+// C [chunkp][Ip] (=, or += with accumulate) sum over pixels of A [chunkp][Pp] (na BF16 terms) and
+// the counts-like operand B [Ip][Pp] (BF16, or bytes when B8 is given): the large-tile kernel.
+hipError_t launch_gemm_store_mfma(const emc_ctx *c, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
+                                  const bf16_t *B16, const uint8_t *B8, float *C, bool accumulate, hipStream_t st);
+// chi^2 logits of rotations [s0, s0+cs) on the MFMA path, from the chunk's FP32 slices (A0 = W,
+// A1 = the pixel weight a): D = sum a W^2 - 2 ip sum a W K + ip^2 sum a K^2 over the valid pixels,
+// ip = 1/phi (options.scaling: sum W K / sum K^2 over W > 1e-10; else 1); R = log w_s - D rscale_s.
+// With scaling, iphi rows [s0, s0+cs) are written for the M-step.
+hipError_t launch_logl_chi2_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st);
+
 // The dense M-step returns at once when gate is non-null and *gate <= gate_thr.
 hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const double *gate, double gate_thr,
                              hipStream_t st);

@@ -49,6 +49,7 @@ struct EpiArgs {
     // once when *gate <= gate_thr (the responsibilities' non-zero count is small)
     const double *gate;
     double gate_thr;
+    bool accumulate;  // store epilogue: C += acc instead of C = acc
 };
 
 __device__ __forceinline__ void scatter_one(const EpiArgs &e, const float *m, float Ts, int j, float u)
@@ -389,7 +390,10 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
             for (int j = 0; j < TN; ++j) {
                 const int n = n0 + wn + j * 32 + (lane & 31);
                 const float v = acc[i][j][r] + add;
-                if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = v;
+                if (n < e.n_valid) {
+                    float *cp = e.C + (size_t)m * e.ldc + n;
+                    *cp = (EPI == EPI_STORE && e.accumulate) ? *cp + v : v;
+                }
                 if (EPI == EPI_LOGL) cmax[j] = fmaxf(cmax[j], v);
             }
         }
@@ -480,7 +484,7 @@ hipError_t run_large(int M, int N, hipStream_t st, int na, const bf16_t *A0, con
 
 }  // namespace
 
-bool logl_fuses_softmax(const emc_ctx *c) { return c->mfma && !c->k_split && c->opt.mfma_tile == 0; }
+bool logl_fuses_softmax(const emc_ctx *c) { return c->mfma && !c->chi2_mfma && !c->k_split && c->opt.mfma_tile == 0; }
 
 int softmax_fused_bands(const emc_ctx *c) { return (int)((int64_t)((c->S + c->chunk - 1) / c->chunk) * (c->chunkp / BM2)); }
 
@@ -511,6 +515,19 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
     }
     if (c->k_split) return run<2, 2, EPI_LOGL>(grid, st, c->Lh, A1, nullptr, c->Pp, c->K16, c->K16lo, c->Pp, c->Pp, e);
     return run<2, 1, EPI_LOGL>(grid, st, c->Lh, A1, nullptr, c->Pp, c->K16, nullptr, c->Pp, c->Pp, e);
+}
+
+hipError_t launch_gemm_store_mfma(const emc_ctx *c, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
+                                  const bf16_t *B16, const uint8_t *B8, float *C, bool accumulate, hipStream_t st)
+{
+    if (na < 2 || na > 3 || c->chunkp % BM2 || c->Ip % 128 || c->Pp % BK2) return hipErrorInvalidValue;
+    EpiArgs e{};
+    e.C = C;
+    e.ldc = c->Ip;
+    e.m_valid = c->chunkp;
+    e.n_valid = c->Ip;
+    e.accumulate = accumulate;
+    return run_large<EPI_STORE>(c->chunkp, c->Ip, st, na, A0, A1, A2, c->Pp, B16, B8, c->Pp, c->Pp, e);
 }
 
 hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const double *gate, double gate_thr,

@@ -93,12 +93,13 @@ __global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, 
 constexpr int kApplyRows = 8;    // rotation rows per workgroup
 constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
 
-// OUT = 0: P in place (FP32).  OUT = 1: BF16 hi/lo pair.
+// OUT = 0: P in place (FP32).  OUT = 1: BF16 hi/lo pair, of P * iphi when iphi is given (the
+// M-step's P/phi under intensity scaling; the slice sums T stay sums of P).
 template <int OUT>
 __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t ldr, int S, int I,
                                                   const double *__restrict__ lse, bf16_t *__restrict__ Ph,
                                                   bf16_t *__restrict__ Pl, double *__restrict__ Tpart,
-                                                  double *__restrict__ nnz)
+                                                  double *__restrict__ nnz, const float *__restrict__ iphi)
 {
     int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
     __shared__ double red[4][kApplyRows];
@@ -126,13 +127,18 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                 *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
             } else {
                 bf16_t h[4], lo[4];
+                const size_t o = (size_t)s * ldr + c0;
+                float q[4] = {p[0], p[1], p[2], p[3]};
+                if (iphi) {
+                    const float4 f = *reinterpret_cast<const float4 *>(iphi + o);
+                    q[0] *= f.x; q[1] *= f.y; q[2] *= f.z; q[3] *= f.w;
+                }
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    h[k] = f2bf(p[k]);
-                    lo[k] = f2bf(p[k] - bf2f(h[k]));
+                    h[k] = f2bf(q[k]);
+                    lo[k] = f2bf(q[k] - bf2f(h[k]));
                     nz += (h[k] | lo[k]) != 0;
                 }
-                const size_t o = (size_t)s * ldr + c0;
                 *reinterpret_cast<uint2 *>(Ph + o) =
                     make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
                 *reinterpret_cast<uint2 *>(Pl + o) =
@@ -219,10 +225,10 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
     double *nzpart = Tpart + (int64_t)ncb * S;  // one non-zero count per apply workgroup
     if (c->mfma)
         hipLaunchKernelGGL(k_sm_apply<1>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, c->Ph, c->Pl,
-                           Tpart, nzpart);
+                           Tpart, nzpart, c->opt.scaling ? c->iphi : nullptr);
     else
         hipLaunchKernelGGL(k_sm_apply<0>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, nullptr, nullptr,
-                           Tpart, nullptr);
+                           Tpart, nullptr, nullptr);
     hipLaunchKernelGGL(k_T_final, dim3((c->Sp + 255) / 256), dim3(256), 0, st, Tpart, ncb, S, c->Sp, c->T);
     if (c->mfma) hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, nzpart, (int)(ga.x * ga.y), c->scal + SC_NNZ);
     return hipGetLastError();

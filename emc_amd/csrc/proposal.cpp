@@ -622,6 +622,7 @@ extern "C" emc_status emc_proposal_enable(emc_ctx *c, const float *coarse, int32
     if (!c->has_rot || !c->has_coords || !c->has_mask || !c->has_K)
         return pfail(c, EMC_E_STATE, "proposal_enable: needs rotations, coords, mask and patterns");
     if (c->k_split) return pfail(c, EMC_E_UNSUPPORTED, "proposal_enable: counts above 256 are not supported");
+    if (c->chi2_mfma) return pfail(c, EMC_E_UNSUPPORTED, "proposal_enable: needs diff = TRUE_POISSON");
     if (o.top_k < 1 || o.top_k > 64 || o.top_k > n_coarse || o.hidden < 64 || o.hidden % 64 || o.hidden > 4096 ||
         o.batch < 64 || o.batch % 64 || o.batch > 65536 || o.epochs < 0 || o.warmup_full < 1 || o.full_every < 1 ||
         !(o.lr > 0.0f) || !(o.min_mass >= 0.0f) || !(o.max_fallback >= 0.0f) || o.synth_patterns < -1 ||

@@ -176,7 +176,8 @@ emc_status emc_model_stats(emc_ctx *ctx, double *max, double *average);
 /* Logits of the last emc_logl, [S][I], constants included (the reference's d_respons before
  * normalisation, emc_cuda.cu:270-275). */
 emc_status emc_get_logl(emc_ctx *ctx, float *logits);
-/* Responsibilities [S][I] after emc_normalize_resp (the isDebug dump, main.cpp:395-417). */
+/* Responsibilities [S][I] after emc_normalize_resp (the isDebug dump, main.cpp:395-417).  With
+ * options.scaling on an MFMA precision the device keeps P / phi for the M-step; this returns P. */
 emc_status emc_get_resp(emc_ctx *ctx, float *resp);
 /* Per-pattern log-sum-exp [I] (main.cpp:384-391, accumulated in FP64). */
 emc_status emc_get_lse(emc_ctx *ctx, double *lse);
@@ -194,7 +195,9 @@ emc_status emc_logl(emc_ctx *ctx, float sigma);
 /* cuda_max_vector + cuda_respons_max_expf + host log/add + cuda_norm_respons_sumexpf
  * (main.cpp:367-392).  total_logl (may be NULL) receives sum_i lse_i. */
 emc_status emc_normalize_resp(emc_ctx *ctx, double *total_logl);
-/* Replaces the responsibilities (for stage tests / external E-steps): [S][I]. */
+/* Replaces the responsibilities (for stage tests / external E-steps): [S][I].  With
+ * options.scaling the M-step then uses P / phi with the phi of the last likelihood pass (1 before
+ * any). */
 emc_status emc_set_resp(emc_ctx *ctx, const float *resp);
 /* cuda_reset_model x2 + chunk loop of cuda_update_slices (main.cpp:443-462). */
 emc_status emc_maximize(emc_ctx *ctx);

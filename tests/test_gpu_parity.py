@@ -119,9 +119,61 @@ def test_scaled_logl_fp32_matches_oracle(orc, small, diff):
 
 
 def test_scaling_rejected_where_unsupported(small):
-    for kw in ({"precision": "bf16x2", "diff": "poisson"}, {"precision": "fp32", "diff": "true_poisson"}):
+    for kw in ({"precision": "bf16x2", "diff": "true_poisson"}, {"precision": "fp32", "diff": "true_poisson"}):
         with pytest.raises(emc_amd.EmcError):
             emc_amd.Engine(small["L"], 100, 50, scaling=1, **kw)
+
+
+@pytest.mark.parametrize("scaling", [0, 1])
+@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
+@pytest.mark.parametrize("diff", ["poisson", "absolute", "relative"])
+def test_chi2_logl_mfma_matches_oracle(orc, small, diff, prec, scaling):
+    """The chi^2 kinds on the matrix cores: D expanded into sum a W^2 - 2 ip sum a W K + ip^2 sum a K^2
+    (split-BF16 slice operands, exact BF16 counts and squared counts), with and without the
+    intensity scaling's two contractions."""
+    log(f"[logl chi2 mfma] diff={diff} prec={prec} scaling={scaling}")
+    d = small
+    S, I = 700, 300
+    sigma = 0.3
+    with engine(d, prec, S=S, I=I, diff=diff, scaling=scaling, slice_chunk=256) as e:
+        e.logl(sigma)
+        R = e.get_logl()
+    M32 = d["m0"].astype(np.float32).astype(np.float64)
+    W = orc.slices(M32, d["L"], d["q"][:S], d["coords"], "trilinear")
+    K = d["K"][:I].astype(np.float64)
+    phi = orc.scaling(W, K, d["mask"]) if scaling else None
+    Ro, _ = orc.logl(W, K, d["mask"], d["w"][:S].astype(np.float64), diff, sigma=sigma, phi=phi)
+    err = rel_logl(R, Ro)
+    log(f"   rel logL err {err:.2e}, max abs err {np.abs(R - Ro).max():.2e}")
+    assert err < (1e-4 if prec == "bf16x2" else 1e-5)
+
+
+@pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
+def test_chi2_mfma_iterations_match_oracle(orc, small, prec):
+    """Two iterations with the reference's own emc.conf choices (chi^2 "poisson", known_intensity =
+    false) on the matrix cores; the M-step contracts the split P / phi."""
+    log(f"[iterate chi2 mfma] prec={prec}")
+    d = small
+    L = d["L"]
+    K = d["K"].astype(np.float64)
+    w64 = d["w"].astype(np.float64)
+    m = d["m0"].astype(np.float32)
+    sig = 0.5
+    with engine(d, prec, diff="poisson", scaling=1, sigma_start=sig, sigma_final=sig, blur_sigma=0.5) as e:
+        for it in range(2):
+            e.set_model(m)
+            st = e.iterate(it)
+            M, _ = e.get_model()
+            P = e.get_resp()
+            ref = orc.iterate(m.astype(np.float64), L, d["q"], w64, K, d["mask"], d["coords"], "trilinear",
+                              "poisson", sigma=sig, blur_sigma=0.5, want_P=True, scaling=True)
+            mt = metrics(None, None, P, ref["P"], M, ref["model"])
+            rel_tot = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
+            log(f"   it {it}: {mt} total logL rel {rel_tot:.2e}")
+            assert rel_tot < 1e-5
+            assert mt["resp_l1_max"] < 5e-3
+            assert mt["model_rel_l2"] < 1e-4
+            m = ref["model"].astype(np.float32)
 
 
 @pytest.mark.parametrize("prec", ["bf16x2", "bf16x3"])
@@ -335,7 +387,7 @@ def test_edge_cases_empty_patterns_and_missing_voxels(orc, small):
 def test_invalid_inputs_are_rejected(small):
     d = small
     with pytest.raises(emc_amd.EmcError):
-        emc_amd.Engine(32, 10, 10, precision="bf16x2", diff="poisson")
+        emc_amd.Engine(32, 10, 10, precision="bf16x2", diff="true_poisson", scaling=1)
     with emc_amd.Engine(32, 60, 10) as e:
         with pytest.raises(emc_amd.EmcError):
             e.logl()  # nothing set
