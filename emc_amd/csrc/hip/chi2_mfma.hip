@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_chi2_operands(const float *__restrict__
         split<NT>(a, t[1][0], t[1][1], t[1][2], o);
         if (scaling) {
             const bool in = W > 1.0e-10f;  // emc_cuda.cu:444 (a zero count adds nothing to either sum)
-            split<NT>(in ? 1.0f : 0.0f, t[2][0], t[2][1], t[2][2], o);
+            t[2][0][o] = f2bf(in ? 1.0f : 0.0f);  // the indicator is exact in one term
             split<NT>(in ? W : 0.0f, t[3][0], t[3][1], t[3][2], o);
         }
     }
@@ -113,18 +113,20 @@ hipError_t launch_logl_chi2_mfma(const emc_ctx *c, int s0, int cs, hipStream_t s
                                          true, st);
         return err;
     };
-    auto with_squares = [&](int p, float *C) {  // C = X_p . (K^2)^T
-        hipError_t err = launch_gemm_store_mfma(c, NT, op(p, 0), op(p, 1), NT == 3 ? op(p, 2) : nullptr, c->K2h,
+    // C = X_p . (K^2)^T with na terms of X_p; the low half of K^2 meets na - 1 of them (at least
+    // one): the dropped product of the two lowest terms is below 2^-16 of the sum's terms.
+    auto with_squares = [&](int p, int na, float *C) {
+        hipError_t err = launch_gemm_store_mfma(c, na, op(p, 0), op(p, 1), na == 3 ? op(p, 2) : nullptr, c->K2h,
                                                 nullptr, C, false, st);
+        const int nl = na > 1 ? na - 1 : 1;
         if (err == hipSuccess && c->k2_split)
-            err = launch_gemm_store_mfma(c, NT, op(p, 0), op(p, 1), NT == 3 ? op(p, 2) : nullptr, c->K2l, nullptr, C,
-                                         true, st);
+            err = launch_gemm_store_mfma(c, nl, op(p, 0), op(p, 1), nullptr, c->K2l, nullptr, C, true, st);
         return err;
     };
     float *R = c->R + (size_t)s0 * c->Ip;
     if (e == hipSuccess) e = with_counts(0, R);
-    if (e == hipSuccess) e = with_squares(1, c->G2);
-    if (e == hipSuccess && scaling) e = with_squares(2, c->G3);
+    if (e == hipSuccess) e = with_squares(1, NT, c->G2);
+    if (e == hipSuccess && scaling) e = with_squares(2, 1, c->G3);
     if (e == hipSuccess && scaling) e = with_counts(3, c->G4);
     if (e != hipSuccess) return e;
     const int64_t n = (int64_t)cs * c->Ip;

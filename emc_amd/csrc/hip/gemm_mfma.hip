@@ -458,9 +458,9 @@ hipError_t run2(int ntm, int ntn, int n_base, hipStream_t st, const bf16_t *A0, 
     return hipGetLastError();
 }
 
-// Large-tile kernel for a single-term B operand (BF16, or bytes when B8 is given).  Two A terms:
-// 256-wide column tiles, plus one 128-wide launch for the remainder when N % 256 = 128; three A
-// terms: 128-wide tiles (LDS).
+// Large-tile kernel for a single-term B operand (BF16, or bytes when B8 is given).  One or two A
+// terms: 256-wide column tiles, plus one 128-wide launch for the remainder when N % 256 = 128;
+// three A terms: 128-wide tiles (LDS).
 template <int EPI, bool U8>
 hipError_t run_large_t(int M, int N, hipStream_t st, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
                        int64_t lda, const void *B, int64_t ldb, int Kd, const EpiArgs &e)
@@ -469,6 +469,12 @@ hipError_t run_large_t(int M, int N, hipStream_t st, int na, const bf16_t *A0, c
     if (na == 3) return run2<3, EPI, 128, U8>(ntm, N / 128, 0, st, A0, A1, A2, lda, B, ldb, Kd, e);
     const int n256 = N / 256;
     hipError_t err = hipSuccess;
+    if (na == 1) {
+        if (n256 > 0) err = run2<1, EPI, 256, U8>(ntm, n256, 0, st, A0, A1, A2, lda, B, ldb, Kd, e);
+        if (err == hipSuccess && N % 256)
+            err = run2<1, EPI, 128, U8>(ntm, 1, n256 * 256, st, A0, A1, A2, lda, B, ldb, Kd, e);
+        return err;
+    }
     if (n256 > 0) err = run2<2, EPI, 256, U8>(ntm, n256, 0, st, A0, A1, A2, lda, B, ldb, Kd, e);
     if (err == hipSuccess && N % 256) err = run2<2, EPI, 128, U8>(ntm, 1, n256 * 256, st, A0, A1, A2, lda, B, ldb, Kd, e);
     return err;
@@ -520,7 +526,7 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
 hipError_t launch_gemm_store_mfma(const emc_ctx *c, int na, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
                                   const bf16_t *B16, const uint8_t *B8, float *C, bool accumulate, hipStream_t st)
 {
-    if (na < 2 || na > 3 || c->chunkp % BM2 || c->Ip % 128 || c->Pp % BK2) return hipErrorInvalidValue;
+    if (na < 1 || na > 3 || c->chunkp % BM2 || c->Ip % 128 || c->Pp % BK2) return hipErrorInvalidValue;
     EpiArgs e{};
     e.C = C;
     e.ldc = c->Ip;
