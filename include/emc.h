@@ -48,7 +48,8 @@ enum { EMC_INTERP_NEAREST = 0, EMC_INTERP_TRILINEAR = 1 };
  *   BF16X2 : MFMA, counts exact in BF16, log W and P each split into BF16 hi+lo, FP32 accumulate.
  *   BF16X3 : as BF16X2 with log W split into three BF16 terms (hi+mid+lo).
  * Counts above 256 are not exact in BF16: the MFMA modes then also split the counts into two
- * exact BF16 terms.  Only EMC_DIFF_TRUE_POISSON runs on MFMA; the chi^2 kinds use FP32. */
+ * exact BF16 terms.  The chi^2 kinds run on MFMA as expanded contractions (sum a W^2 - 2 sum a W K
+ * + sum a K^2, with K^2 as two exact BF16 terms) and need counts <= 255 there. */
 enum { EMC_PREC_FP32 = 0, EMC_PREC_BF16X2 = 1, EMC_PREC_BF16X3 = 2 };
 
 /* How the M-step inserts the weighted slice sums into the model.
@@ -95,8 +96,8 @@ typedef struct emc_options {
                                 sum K^2 / sum K W over {mask, W > 1e-10, K > 1e-10}
                                 (emc_cuda.cu:438-454, 516-532), the chi^2 kinds compare W with K/phi
                                 and the M-step sums K P / phi (emc_atomic.cu:106-108); both sums
-                                of phi are contractions over pixels.  FP32 chi^2 kinds only.
-                                0 = phi = 1 (known_intensity = true) */
+                                of phi are contractions over pixels.  chi^2 kinds only (any
+                                precision).  0 = phi = 1 (known_intensity = true) */
     int32_t reserved[3];
 } emc_options;
 
