@@ -276,7 +276,11 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN2 / 2);
-    const int lrow = tid >> 2, lcol = (tid & 3) * 8;  // one 16-byte chunk: row, k offset
+    // One 16-byte chunk per thread and 128-row half: k offset lcol of row lrow.  Lanes 4q..4q+3
+    // hold the four chunks of one row, and the two rows of a lane octet are 4 apart (rows 8k + j
+    // and 8k + j + 4): with 80-byte LDS rows the octet's stores then fall on 8 distinct 16-byte
+    // bank groups (adjacent rows would share one).
+    const int lrow = ((tid >> 5) << 3) + ((tid >> 3) & 3) + (((tid >> 2) & 1) << 2), lcol = (tid & 3) * 8;
 
     const bf16_t *ga[NA];
     ga[0] = A0 + (size_t)(m0 + lrow) * lda + lcol;
