@@ -202,7 +202,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
     dfree(c->Cx); dfree(c->K2h); dfree(c->K2l); dfree(c->Aw); dfree(c->G2); dfree(c->G3); dfree(c->G4);
-    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
+    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->pdirty); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
     dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
@@ -280,6 +280,8 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK && c->la_terms == 3) A(dalloc(&c->Lm, cp * PpMax), "Lm");
         if (st == EMC_OK) A(dalloc(&c->Ph, Sp * Ip), "Ph");
         if (st == EMC_OK) A(dalloc(&c->Pl, Sp * Ip), "Pl");
+        if (st == EMC_OK) A(dalloc(&c->pdirty, Sp * (int64_t)p_segments(c)), "pdirty");
+        if (st == EMC_OK) A(hipMemset(c->pdirty, 1, sizeof(uint8_t) * Sp * p_segments(c)), "pdirty");
         if (st == EMC_OK) A(dalloc(&c->colsum, 2 * (int64_t)PpMax), "colsum");
         if (st == EMC_OK) A(dalloc(&c->lw32, Sp * PpMax), "lw32");
         if (c->chi2_mfma) {
@@ -789,6 +791,7 @@ extern "C" emc_status emc_set_resp(emc_ctx *c, const float *resp)
                                hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy2D(c->Pl, sizeof(bf16_t) * c->Ip, l.data(), sizeof(bf16_t) * I, sizeof(bf16_t) * I, S,
                                hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemset(c->pdirty, 1, sizeof(uint8_t) * c->Sp * p_segments(c)));
     }
     HIP_TRY(c, hipMemcpy(c->T, T.data(), sizeof(double) * c->Sp, hipMemcpyHostToDevice));
     c->has_resp = true;

@@ -176,6 +176,7 @@ struct emc_ctx {
     // responsibilities
     float *R = nullptr;                                // [Sp][Ip] logits (P in place on FP32 path)
     emc::bf16_t *Ph = nullptr, *Pl = nullptr;          // [Sp][Ip] BF16 hi/lo P (MFMA path)
+    uint8_t *pdirty = nullptr;  // [Sp][ceil(I/256)]: 0 = that 256-pattern segment of Ph/Pl holds only zeros
     double *lse = nullptr;                             // [Ip] lse' (constants excluded)
     double *part = nullptr;                            // softmax / reduction partials
     int64_t part_len = 0;
@@ -238,6 +239,10 @@ hipError_t launch_gemm_store_mfma(const emc_ctx *c, int na, const bf16_t *A0, co
 // ip = 1/phi (options.scaling: sum W K / sum K^2 over W > 1e-10; else 1); R = log w_s - D rscale_s.
 // With scaling, iphi rows [s0, s0+cs) are written for the M-step.
 hipError_t launch_logl_chi2_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st);
+// Ph / Pl segments of 256 patterns tracked by pdirty, and marking every one as possibly non-zero
+// (after anything but the dense softmax wrote Ph / Pl).
+inline int p_segments(const emc_ctx *c) { return (c->I + 255) / 256; }
+hipError_t launch_mark_p_dirty(const emc_ctx *c, hipStream_t st);
 
 // The dense M-step returns at once when gate is non-null and *gate <= gate_thr.
 hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const double *gate, double gate_thr,

@@ -489,6 +489,8 @@ hipError_t launch_grouped_softmax(const emc_ctx *c, hipStream_t st)
     double *nzrow = c->part + blk_pad(c);  // one non-zero count per fine rotation
     hipLaunchKernelGGL(k_gsm_apply, dim3((c->S + 3) / 4), dim3(256), 0, st, c->R, g, c->S, p->slot_lse, c->Ph, c->Pl,
                        c->T, nzrow);
+    e = launch_mark_p_dirty(c, st);  // Ph / Pl now hold the grouped layout
+    if (e != hipSuccess) return e;
     return launch_sum_f64(nzrow, c->S, c->scal + SC_NNZ, st);
 }
 

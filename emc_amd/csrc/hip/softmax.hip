@@ -95,16 +95,22 @@ constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
 
 // OUT = 0: P in place (FP32).  OUT = 1: BF16 hi/lo pair, of P * iphi when iphi is given (the
 // M-step's P/phi under intensity scaling; the slice sums T stay sums of P).
+// OUT = 1 skips the store of a wave's 256-pattern row segment when every value in it is zero and
+// pdirty says the segment already holds only zeros (P is mostly zeros once the posteriors are
+// sharp); pdirty is then updated to whether the segment holds a non-zero value.
 template <int OUT>
 __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t ldr, int S, int I,
                                                   const double *__restrict__ lse, bf16_t *__restrict__ Ph,
                                                   bf16_t *__restrict__ Pl, double *__restrict__ Tpart,
-                                                  double *__restrict__ nnz, const float *__restrict__ iphi)
+                                                  double *__restrict__ nnz, const float *__restrict__ iphi,
+                                                  uint8_t *__restrict__ pdirty, int nseg)
 {
     int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
     __shared__ double red[4][kApplyRows];
     const int c0 = blockIdx.x * kApplyCols + threadIdx.x * 4;
     const int r0 = blockIdx.y * kApplyRows;
+    const int seg = (blockIdx.x * kApplyCols) / 256 + (threadIdx.x >> 6);  // this wave's 256 patterns
+    const bool seg_live = seg * 256 < I;                                    // uniform over the wave
     double l[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) l[k] = (c0 + k < I) ? lse[c0 + k] : 0.0;
@@ -113,6 +119,8 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
     for (int rr = 0; rr < kApplyRows; ++rr) {
         const int s = r0 + rr;
         double acc = 0.0;
+        uint2 hv = make_uint2(0u, 0u), lv = make_uint2(0u, 0u);
+        int any_nz = 0;
         if (s < S && c0 < I) {
             float *src = R + (size_t)s * ldr + c0;
             const float4 v = *reinterpret_cast<const float4 *>(src);
@@ -133,17 +141,29 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                     const float4 f = *reinterpret_cast<const float4 *>(iphi + o);
                     q[0] *= f.x; q[1] *= f.y; q[2] *= f.z; q[3] *= f.w;
                 }
+                int mine = 0;
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     h[k] = f2bf(q[k]);
                     lo[k] = f2bf(q[k] - bf2f(h[k]));
-                    nz += (h[k] | lo[k]) != 0;
+                    mine += (h[k] | lo[k]) != 0;
                 }
-                *reinterpret_cast<uint2 *>(Ph + o) =
-                    make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
-                *reinterpret_cast<uint2 *>(Pl + o) =
-                    make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+                nz += mine;
+                hv = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+                lv = make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+                any_nz = mine != 0;
             }
+        }
+        if (OUT == 1 && s < S && seg_live) {  // s and seg_live are uniform over the wave
+            const bool seg_nz = __any(any_nz) != 0;
+            const size_t fo = (size_t)s * nseg + seg;
+            const bool dirty = pdirty[fo] != 0;
+            if ((seg_nz || dirty) && c0 < I) {
+                const size_t o = (size_t)s * ldr + c0;
+                *reinterpret_cast<uint2 *>(Ph + o) = hv;
+                *reinterpret_cast<uint2 *>(Pl + o) = lv;
+            }
+            if ((threadIdx.x & 63) == 0 && dirty != seg_nz) pdirty[fo] = seg_nz ? 1 : 0;
         }
         rowsum[rr] = acc;
     }
@@ -198,6 +218,21 @@ int64_t softmax_part_len(int S, int I, int fused_bands)
            (int64_t)ncb * S + (int64_t)ncb * ((S + kApplyRows - 1) / kApplyRows) + 16;
 }
 
+__global__ void k_mark_dirty(uint8_t *__restrict__ d, int64_t n)
+{
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+        d[t] = 1;
+}
+
+hipError_t launch_mark_p_dirty(const emc_ctx *c, hipStream_t st)
+{
+    if (!c->pdirty) return hipSuccess;
+    const int64_t n = (int64_t)c->Sp * p_segments(c);
+    hipLaunchKernelGGL(k_mark_dirty, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, st,
+                       c->pdirty, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_sum_f64(const double *x, int n, double *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, x, n, out);
@@ -225,10 +260,10 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
     double *nzpart = Tpart + (int64_t)ncb * S;  // one non-zero count per apply workgroup
     if (c->mfma)
         hipLaunchKernelGGL(k_sm_apply<1>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, c->Ph, c->Pl,
-                           Tpart, nzpart, c->opt.scaling ? c->iphi : nullptr);
+                           Tpart, nzpart, c->opt.scaling ? c->iphi : nullptr, c->pdirty, p_segments(c));
     else
         hipLaunchKernelGGL(k_sm_apply<0>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, nullptr, nullptr,
-                           Tpart, nullptr, nullptr);
+                           Tpart, nullptr, nullptr, nullptr, 0);
     hipLaunchKernelGGL(k_T_final, dim3((c->Sp + 255) / 256), dim3(256), 0, st, Tpart, ncb, S, c->Sp, c->T);
     if (c->mfma) hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, nzpart, (int)(ga.x * ga.y), c->scal + SC_NNZ);
     return hipGetLastError();
