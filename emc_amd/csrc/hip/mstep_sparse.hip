@@ -19,11 +19,13 @@ __global__ __launch_bounds__(256) void k_mstep_sparse(const bf16_t *__restrict__
                                                       int64_t ldp, int s0, int I, const void *__restrict__ Kv,
                                                       const bf16_t *__restrict__ Klo, int Pp,
                                                       const double *__restrict__ gate, double gate_thr,
-                                                      float *__restrict__ U)
+                                                      float *__restrict__ U, const uint8_t *__restrict__ pdirty,
+                                                      int nseg)
 {
     if (gate && *gate > gate_thr) return;  // the dense contraction runs instead
     const size_t o = (size_t)(s0 + blockIdx.x) * ldp;
-    sparse_row<KT, NPASS>(Ph + o, Pl + o, I, nullptr, Kv, Klo, Pp, U + (size_t)blockIdx.x * Pp);
+    sparse_row<KT, NPASS>(Ph + o, Pl + o, I, nullptr, Kv, Klo, Pp, U + (size_t)blockIdx.x * Pp,
+                          pdirty ? pdirty + (size_t)(s0 + blockIdx.x) * nseg : nullptr);
 }
 
 }  // namespace
@@ -34,13 +36,13 @@ static void launch_np(const emc_ctx *c, int s0, int cs, const double *gate, doub
     const dim3 grid(cs), blk(256);
     if (c->K8)
         hipLaunchKernelGGL((k_mstep_sparse<0, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K8,
-                           nullptr, c->Pp, gate, gate_thr, c->U);
+                           nullptr, c->Pp, gate, gate_thr, c->U, c->pdirty, p_segments(c));
     else if (c->K16lo)
         hipLaunchKernelGGL((k_mstep_sparse<2, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16,
-                           c->K16lo, c->Pp, gate, gate_thr, c->U);
+                           c->K16lo, c->Pp, gate, gate_thr, c->U, c->pdirty, p_segments(c));
     else
         hipLaunchKernelGGL((k_mstep_sparse<1, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16,
-                           nullptr, c->Pp, gate, gate_thr, c->U);
+                           nullptr, c->Pp, gate, gate_thr, c->U, c->pdirty, p_segments(c));
 }
 
 hipError_t launch_mstep_sparse(const emc_ctx *c, int s0, int cs, const double *gate, double gate_thr, hipStream_t st)

@@ -22,6 +22,7 @@ constexpr int kCap = 2048;        // LDS list entries per batch
 constexpr int kV = 4;             // 16-byte loads of hi per thread and step
 constexpr int kPer = 8 * kV;      // consecutive entries per thread and step
 constexpr int kStep = 256 * kPer; // entries scanned per step
+static_assert(256 % kPer == 0, "a thread's entries lie in one 256-entry segment");
 
 __device__ __forceinline__ float half16(const u32x4 &v, int k) { return bf2f((bf16_t)(v[k >> 1] >> (16 * (k & 1)))); }
 __device__ __forceinline__ bool nonzero(const u32x4 &v) { return (v[0] | v[1] | v[2] | v[3]) != 0u; }
@@ -29,11 +30,13 @@ __device__ __forceinline__ bool nonzero(const u32x4 &v) { return (v[0] | v[1] | 
 
 // n entries of the row; pat == nullptr: entry n is pattern n.  Pp is a multiple of 16 and at most
 // 4096 * NPASS; the row holds a multiple of 8 readable entries (padding entries are 0) and starts
-// 16-byte aligned.
+// 16-byte aligned.  seg (or nullptr): one flag per 256 entries of the row, 0 where those entries are
+// known to be zero (their loads are skipped).
 template <int KT, int NPASS>
 __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const bf16_t *__restrict__ pl, int n,
                                            const int32_t *__restrict__ pat, const void *__restrict__ Kv,
-                                           const bf16_t *__restrict__ Klo, int Pp, float *__restrict__ out)
+                                           const bf16_t *__restrict__ Klo, int Pp, float *__restrict__ out,
+                                           const uint8_t *__restrict__ seg = nullptr)
 {
     using namespace sparse_row_detail;
     __shared__ int li[kCap];
@@ -84,14 +87,15 @@ __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const 
 
     int m = 0;  // entries in the list (the same in every thread)
     for (int i0 = 0; i0 < n; i0 += kStep) {
-        const int ib = i0 + tid * kPer;
+        const int ib = i0 + tid * kPer;  // kPer divides 256: the thread's entries share one flag
+        const bool live = ib < n && (!seg || seg[ib >> 8] != 0);
         u32x4 h[kV], l[kV];
         bool any = false;
 #pragma unroll
         for (int v = 0; v < kV; ++v) {
             h[v] = u32x4{0u, 0u, 0u, 0u};
             l[v] = u32x4{0u, 0u, 0u, 0u};
-            if (ib + 8 * v < n) h[v] = *reinterpret_cast<const u32x4 *>(ph + ib + 8 * v);
+            if (live && ib + 8 * v < n) h[v] = *reinterpret_cast<const u32x4 *>(ph + ib + 8 * v);
             any |= nonzero(h[v]);
         }
         if (!__syncthreads_or(any)) continue;
