@@ -81,8 +81,12 @@ def stage_rooflines(st, S, I, Pa, L, ms_iter):
         # name: (bound, units, per-iteration amount, stage time in ms)
         "slice": ("hbm", "B", S * Pa * 4 + N3 * 4, st["slice"] + (st["logl"] - st["logl_gemm"])),
         "logl_contraction": ("mfma", "FLOP", 2.0 * I * S * Pa, st["logl_gemm"]),
-        "softmax": ("hbm", "B", S * I * 4 + S * I * 4, st["softmax"]),
-        "mstep_sparse_scan": ("hbm", "B", S * I * 2, st["mstep_gemm"]),
+        # logits read once; P is stored only for its segments that hold non-zeros, so the dense
+        # write is not counted
+        "softmax": ("hbm", "B", S * I * 4, st["softmax"]),
+        # sparse M-step: the segment flags scanned and U written (the non-zero P entries and their
+        # count rows are a small fraction of it)
+        "mstep_sparse": ("hbm", "B", S * ((I + 255) // 256) + S * Pa * 4, st["mstep_gemm"]),
         "insertion": ("hbm", "B", S * Pa * 4 + 2 * N3 * 4, t_ins),
         "compress_blur_normalise": ("hbm", "B", N3 * 4 * (3 + 2 * 3 + 2), st["compress"]),
     }

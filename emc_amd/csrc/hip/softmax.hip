@@ -111,9 +111,15 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
     const int r0 = blockIdx.y * kApplyRows;
     const int seg = (blockIdx.x * kApplyCols) / 256 + (threadIdx.x >> 6);  // this wave's 256 patterns
     const bool seg_live = seg * 256 < I;                                    // uniform over the wave
-    double l[4];
+    // lse as a float pair: x - lh is exact wherever p is not negligible (x and lse within a factor
+    // of 2, Sterbenz), so exp((x - lh) - ll) carries the double-precision argument to float rounding
+    float lh[4], ll[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) l[k] = (c0 + k < I) ? lse[c0 + k] : 0.0;
+    for (int k = 0; k < 4; ++k) {
+        const double l = (c0 + k < I) ? lse[c0 + k] : 0.0;
+        lh[k] = (float)l;
+        ll[k] = (float)(l - (double)lh[k]);
+    }
     double rowsum[kApplyRows];
 #pragma unroll
     for (int rr = 0; rr < kApplyRows; ++rr) {
@@ -127,10 +133,8 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
             const float x[4] = {v.x, v.y, v.z, v.w};
             float p[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                p[k] = (c0 + k < I) ? expf((float)((double)x[k] - l[k])) : 0.0f;
-                acc += (double)p[k];
-            }
+            for (int k = 0; k < 4; ++k) p[k] = (c0 + k < I) ? expf((x[k] - lh[k]) - ll[k]) : 0.0f;
+            acc = (double)((p[0] + p[1]) + (p[2] + p[3]));
             if (OUT == 0) {
                 *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
             } else {
