@@ -293,7 +293,8 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
 // cst_i = sum_j K_ij colmean_j - B0   (FP64; one workgroup per pattern)
 __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__ K32,
                                                        const bf16_t *__restrict__ K16,
-                                                       const bf16_t *__restrict__ K16lo, int I, int Pp,
+                                                       const bf16_t *__restrict__ K16lo,
+                                                       const uint8_t *__restrict__ K8, int I, int Pp,
                                                        const float *__restrict__ colmean,
                                                        const double *__restrict__ sumK,
                                                        const double *__restrict__ scal, bool centred,
@@ -302,7 +303,20 @@ __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__
     __shared__ double red[4];
     const int i = blockIdx.x;
     double acc = 0.0;
-    if (centred) {
+    if (centred && K8) {  // byte counts (exact): 16 per thread and 16-byte load (Pp % 16 == 0)
+        for (int j = threadIdx.x * 16; j < Pp; j += 256 * 16) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(K8 + (size_t)i * Pp + j);
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 m = *reinterpret_cast<const float4 *>(colmean + j + 4 * q);
+                acc = fma((double)(w[q] & 0xffu), (double)m.x, acc);
+                acc = fma((double)((w[q] >> 8) & 0xffu), (double)m.y, acc);
+                acc = fma((double)((w[q] >> 16) & 0xffu), (double)m.z, acc);
+                acc = fma((double)(w[q] >> 24), (double)m.w, acc);
+            }
+        }
+    } else if (centred) {
         for (int j = threadIdx.x; j < Pp; j += 256) {
             float k;
             if (K32) k = K32[(size_t)i * Pp + j];
@@ -372,7 +386,7 @@ hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st)
 
 hipError_t launch_pattern_const(const emc_ctx *c, bool centred, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pattern_const, dim3(c->I), dim3(256), 0, st, c->K32, c->K16, c->K16lo, c->I,
+    hipLaunchKernelGGL(k_pattern_const, dim3(c->I), dim3(256), 0, st, c->K32, c->K16, c->K16lo, c->K8, c->I,
                        c->Pp, c->colmean, c->sumK, c->scal, centred, c->cst);
     return hipGetLastError();
 }

@@ -49,21 +49,33 @@ __global__ __launch_bounds__(256) void k_sm_partial(const float *__restrict__ R,
     ps[(size_t)band * I + i] = sum;
 }
 
+// lse_i = m_i + log sum_b ps_b exp(pm_b - m_i) over the bands.  Eight lanes per pattern take every
+// eighth band and combine over a fixed butterfly (deterministic); 32 patterns per workgroup.
+constexpr int kFinalPat = 32;
 __global__ __launch_bounds__(256) void k_sm_final(const float *__restrict__ pm, const double *__restrict__ ps,
                                                   int nbands, int I, double *__restrict__ lse,
                                                   double *__restrict__ blk_sum, double *__restrict__ scal)
 {
     __shared__ double red[4];
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    double l = 0.0;
-    if (i < I) {
-        float m = -1.0e10f;
-        for (int b = 0; b < nbands; ++b) m = fmaxf(m, pm[(size_t)b * I + i]);
-        double sum = 0.0;
-        for (int b = 0; b < nbands; ++b) {
+    const int g = threadIdx.x & 7;
+    const int i = blockIdx.x * kFinalPat + (threadIdx.x >> 3);
+    float m = -1.0e10f;
+    if (i < I)
+        for (int b = g; b < nbands; b += 8) m = fmaxf(m, pm[(size_t)b * I + i]);
+    m = fmaxf(m, __shfl_xor(m, 1, 64));
+    m = fmaxf(m, __shfl_xor(m, 2, 64));
+    m = fmaxf(m, __shfl_xor(m, 4, 64));
+    double sum = 0.0;
+    if (i < I)
+        for (int b = g; b < nbands; b += 8) {
             const double sb = ps[(size_t)b * I + i];
             if (sb > 0.0) sum += sb * exp((double)pm[(size_t)b * I + i] - (double)m);
         }
+    sum += __shfl_xor(sum, 1, 64);
+    sum += __shfl_xor(sum, 2, 64);
+    sum += __shfl_xor(sum, 4, 64);
+    double l = 0.0;
+    if (i < I && g == 0) {
         l = (double)m + log(sum);
         if (!isfinite(l)) scal[SC_NONFINITE] = 1.0;
         lse[i] = l;
@@ -218,7 +230,7 @@ int64_t softmax_part_len(int S, int I, int fused_bands)
 {
     const int64_t nb = std::max(softmax_bands(S, I), fused_bands);
     const int ncb = (I + kApplyCols - 1) / kApplyCols;
-    return nb * I /*pm as floats packed in doubles: over-allocate*/ + nb * I + (I + 255) / 256 + 1 +
+    return nb * I /*pm as floats packed in doubles: over-allocate*/ + nb * I + (I + kFinalPat - 1) / kFinalPat + 1 +
            (int64_t)ncb * S + (int64_t)ncb * ((S + kApplyRows - 1) / kApplyRows) + 16;
 }
 
@@ -247,6 +259,7 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
 {
     const int S = c->S, I = c->I;
     const int ncol = (I + 255) / 256;
+    const int nfin = (I + kFinalPat - 1) / kFinalPat;
     const bool fused = logl_fuses_softmax(c);  // the logL epilogue wrote the band partials
     const int nb = fused ? softmax_fused_bands(c) : softmax_bands(S, I);
     const int rows = (S + nb - 1) / nb;
@@ -254,11 +267,11 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
     float *pm = reinterpret_cast<float *>(base);
     double *ps = base + (int64_t)nb * I;
     double *blk = ps + (int64_t)nb * I;
-    double *Tpart = blk + ncol + 1;
+    double *Tpart = blk + nfin + 1;
     if (!fused)
         hipLaunchKernelGGL(k_sm_partial, dim3(ncol, nb), dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, rows, pm, ps);
-    hipLaunchKernelGGL(k_sm_final, dim3(ncol), dim3(256), 0, st, pm, ps, nb, I, c->lse, blk, c->scal);
-    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, blk, ncol, c->scal + SC_TOTAL);
+    hipLaunchKernelGGL(k_sm_final, dim3(nfin), dim3(256), 0, st, pm, ps, nb, I, c->lse, blk, c->scal);
+    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, blk, nfin, c->scal + SC_TOTAL);
     const int ncb = (I + kApplyCols - 1) / kApplyCols;
     const dim3 ga(ncb, (S + kApplyRows - 1) / kApplyRows);
     double *nzpart = Tpart + (int64_t)ncb * S;  // one non-zero count per apply workgroup
