@@ -334,5 +334,10 @@ __host__ __device__ inline bf16_t f2bf(float x)
     return (bf16_t)(u >> 16);
 }
 __host__ __device__ inline float bf2f(bf16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+// The same rounding on the gfx950 conversion instruction (one v_cvt_pk_bf16_f32 per value; equal to
+// f2bf for every non-NaN float, checked over all 2^32 bit patterns on the device).  Scalar form only:
+// the two-element vector conversion, __builtin_convertvector(float2 -> __bf16 x 2), is lowered to one
+// instruction that converts the first element alone.
+__device__ inline bf16_t f2bf_dev(float x) { return __builtin_bit_cast(bf16_t, (__bf16)x); }
 
 }  // namespace emc

@@ -317,8 +317,12 @@ __global__ __launch_bounds__(256) void k_gsm_apply(const float *__restrict__ Lb,
     int nz = 0;  // non-zero P of the row (for the choice of the M-step form)
     for (int s = lane; s < nsp; s += 64) {
         float p = 0.0f;
-        if (s < ns) p = expf((float)((double)Lb[row + s] - slse[sb + s]));
-        const bf16_t h = f2bf(p), lo = f2bf(p - bf2f(h));
+        if (s < ns) {  // the dense softmax's arithmetic: lse as a float pair, exp on the hardware exp2
+            const double l = slse[sb + s];
+            const float lh = (float)l, ll = (float)(l - (double)lh);
+            p = __builtin_amdgcn_exp2f(((Lb[row + s] - lh) - ll) * 1.4426950408889634f);
+        }
+        const bf16_t h = f2bf_dev(p), lo = f2bf_dev(p - bf2f(h));
         Ph[row + s] = h;
         Pl[row + s] = lo;
         acc += (double)p;

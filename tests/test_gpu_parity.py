@@ -6,6 +6,7 @@ import pytest
 
 import emc_amd
 from emc_amd import workloads as wl
+from conftest import rel_up_to_normaliser
 
 pytestmark = pytest.mark.gpu
 
@@ -32,18 +33,6 @@ def metrics(R_gpu, R_orc, P_gpu, P_orc, M_gpu, M_orc):
     if M_gpu is not None:
         out["model_rel_l2"] = float(np.linalg.norm(M_gpu - M_orc) / np.linalg.norm(M_orc))
     return out
-
-
-def rel_up_to_normaliser(Ma, Mb):
-    """(relative L2 difference after matching the global scale, relative scale difference).
-    The model is divided by its mean over positive voxels; after the blur, voxels far from any
-    data hold values at the rounding level whose sign can follow the (unordered) float atomics of
-    the insertion, so the positive-voxel count, and with it the scale, can move by 1/count."""
-    top = max(float(Ma.max()), float(Mb.max()))
-    sel = (Ma > 1e-3 * top) & (Mb > 1e-3 * top)
-    c = float(Mb[sel].astype(np.float64).sum() / Ma[sel].astype(np.float64).sum())
-    rel = float(np.linalg.norm(c * Ma.astype(np.float64) - Mb) / np.linalg.norm(Mb))
-    return rel, abs(c - 1.0)
 
 
 @pytest.fixture(scope="module")
