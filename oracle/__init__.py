@@ -7,8 +7,9 @@ this package, and only as the checker (or as the timed CPU baseline).  The produ
 The arithmetic lives in ``emc_oracle.c`` (built to ``liboracle.so`` by ``make -C oracle``); this
 module is a thin numpy/ctypes veneer.  Every function names the reference lines it restates in the
 C source.  Parity status: every function here is pinned by the checks in ``tests/test_oracle_*.py``
-(closed forms, brute force, invariants, independent library routines); the reference itself is
-unbuildable here and ships no tests or golden vectors.
+(closed forms, brute force, invariants, independent library routines); the reference's kernels are
+CUDA (no nvcc or CUDA headers here), so the reference itself cannot be built as a second checker, and it
+ships no tests or golden vectors for this path.
 """
 from __future__ import annotations
 
