@@ -9,7 +9,11 @@ beside it (`fp32`), and the FP64 CPU oracle on a rotation subset is the `cpu_bas
 and 50,000 patterns (64^3 model, 2,000 photons): the same iterations from the same start.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload NAME] [--no-cpu] [--no-fp32]
-                    [--no-proposal] [--proposal-iters 48] [--accuracy] [--variants]
+                    [--no-proposal] [--proposal-iters 48] [--accuracy] [--variants] [--detail FILE]
+
+The last stdout line is one compact JSON object (headline, roofline of the logL contraction, CPU
+baseline, FP32 and chi^2 legs, proposal summary).  Progress and per-iteration rows go to stderr;
+--detail writes every measurement, per-iteration rows included, to FILE.
 """
 from __future__ import annotations
 
@@ -340,13 +344,42 @@ def run_proposal(name: str, iters: int, options: dict | None = None, diagnose: b
     return out
 
 
+def proposal_summary(prop):
+    """The proposal comparison reduced to a few numbers for the bench line (the per-iteration rows
+    go to stderr and to the --detail file)."""
+    if not prop or "error" in prop:
+        return prop
+    r3 = lambda v: None if v is None else round(float(v), 3)
+    rows = prop.get("per_iteration", [])
+    keys = ("full_search_ms", "proposal_iteration_ms", "full_with_training_ms", "speedup_proposal_iteration",
+            "speedup_amortised", "speedup_run_after_warmup")
+    out = {k: r3(prop.get(k)) for k in keys}
+    out.update({
+        "workload": prop["workload"], "iterations": prop["iterations"],
+        "captured_mass_after_proposals": [round(v, 5) for v in prop["captured_mass_after_proposals"]],
+        "min_captured_after_proposals": (round(min(prop["captured_mass_after_proposals"]), 5)
+                                         if prop["captured_mass_after_proposals"] else None),
+        "final_model_rel_l2_vs_full": float(f"{prop['final_model_rel_l2_vs_full']:.3e}"),
+        "mean_work_fraction_proposal": r3(np.mean([r["work_fraction"] for r in rows if r["search"] == "proposal"])
+                                          if any(r["search"] == "proposal" for r in rows) else None),
+        "full_searches": [r["it"] for r in rows if r["search"] == "full"],
+        "full_searches_retrained": prop.get("full_searches_retrained"),
+    })
+    if "second_half" in prop:
+        sh = prop["second_half"]
+        out["second_half"] = {"iterations": sh["iterations"], "speedup_run": r3(sh["speedup_run"]),
+                              "speedup_proposal_iteration": r3(sh["speedup_proposal_iteration"])}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="L64_S25680_I20000")
-    ap.add_argument("--fp32-steps", type=int, default=2)
+    ap.add_argument("--fp32-steps", type=int, default=5, help="timed iterations of the plain FP32 path")
+    ap.add_argument("--chi2-steps", type=int, default=2, help="timed iterations of the chi^2 + scaling legs")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
@@ -361,6 +394,8 @@ def main():
     ap.add_argument("--proposal-iters", type=int, default=48)
     ap.add_argument("--proposal-options", default="{}", help="JSON object of emc_proposal_options overrides")
     ap.add_argument("--diagnose", action="store_true", help="proposal: where the missed responsibility mass went")
+    ap.add_argument("--detail", default="", help="also write every measurement (per-iteration rows included) "
+                                                 "as JSON to this file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -373,8 +408,12 @@ def main():
     info = emc_amd.device_info(0)
     log(f"[bench] device {info['name']}, {info['n_cu']} CUs, {info['hbm_bytes'] / 2**30:.0f} GiB")
     if args.proposal_only:
-        print(json.dumps({"proposal": run_proposal("L64_S204960_I50000", args.proposal_iters,
-                                                   json.loads(args.proposal_options), args.diagnose)}))
+        prop = run_proposal("L64_S204960_I50000", args.proposal_iters, json.loads(args.proposal_options),
+                            args.diagnose)
+        if args.detail:
+            with open(args.detail, "w") as f:
+                json.dump({"proposal": prop}, f)
+        print(json.dumps({"proposal": proposal_summary(prop)}))
         return
     t = time.perf_counter()
     d = wl.make(args.workload)
@@ -387,13 +426,13 @@ def main():
     fp32 = None
     if not args.no_fp32:
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
-        fp32 = {"value": 1000.0 / ms32, "ms_per_step": ms32, "stage_ms": st32,
+        fp32 = {"value": 1000.0 / ms32, "ms_per_step": ms32, "steps": args.fp32_steps, "stage_ms": st32,
                 "logl_gemm_tflops": 2.0 * I * S * Pa / (st32["logl_gemm"] * 1e-3) / 1e12}
         # the reference's own emc.conf choices: chi^2 ("poisson") with known_intensity = false, on
         # the plain FP32 path and on the matrix cores (expanded contractions)
-        mss, sts, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic", diff="poisson", scaling=1)
+        mss, sts, _ = run_engine(d, "fp32", args.chi2_steps, 1, "atomic", diff="poisson", scaling=1)
         fp32["chi2_scaled"] = {"value": 1000.0 / mss, "ms_per_step": mss, "stage_ms": sts}
-        msm, stm, _ = run_engine(d, "bf16x2", args.steps, 1, args.insert, diff="poisson", scaling=1)
+        msm, stm, _ = run_engine(d, "bf16x2", args.chi2_steps, 1, args.insert, diff="poisson", scaling=1)
         fp32["chi2_scaled_mfma"] = {"value": 1000.0 / msm, "ms_per_step": msm, "stage_ms": stm,
                                     "speedup_vs_fp32": mss / msm}
 
@@ -410,11 +449,12 @@ def main():
     flops = 2.0 * I * S * Pa  # one logical K.logW^T product, counted once (the BF16 split is overhead)
     t_logl = st_mfma["logl_gemm"] * 1e-3
     achieved = flops / t_logl / 1e12
-    roof = {"kernel": "logl_mfma (k_gemm_mfma2<2, 0, 256, true>)", "bound": "mfma", "achieved": achieved,
-            "peak": PEAK["bf16_tflops"], "unit": "TFLOP/s", "frac": achieved / PEAK["bf16_tflops"],
+    roof = {"kernel": "k_gemm_mfma2 (logL contraction, 256x256 tiles, byte counts)", "bound": "mfma",
+            "achieved": achieved, "peak": PEAK["bf16_tflops"], "unit": "TFLOP/s",
+            "frac": achieved / PEAK["bf16_tflops"],
             "traffic": traffic_from_profile("logl_mfma", args.workload),
-            "algorithmic": f"2*I*S*P_act = {flops:.3e} FLOP per launch (one launch per iteration)",
-            "mstep_achieved": flops / (st_mfma["mstep_gemm"] * 1e-3) / 1e12}
+            "algorithmic": f"2*I*S*P_act = {flops:.4e} FLOP per launch (one launch per iteration)",
+            "launch_ms": st_mfma["logl_gemm"]}
 
     stages = stage_rooflines(st_mfma, S, I, Pa, L, ms_mfma)
     log(f"[bench] stage rooflines {json.dumps(stages)}")
@@ -439,34 +479,49 @@ def main():
             log(f"[bench] cpu baseline unavailable: {ex}")
 
     value = 1000.0 / ms_mfma
+    r3 = lambda v: round(float(v), 3)
+    config = {"workload": args.workload, "model_side": L, "rotations": S, "patterns": I, "active_pixels": Pa,
+              "photons": d["cfg"]["photons"], "precision": "bf16x2", "insert": args.insert, "mstep": args.mstep,
+              "interp": "trilinear", "likelihood": "true_poisson", "blur_sigma": d["cfg"]["blur"],
+              "parallelism": "single_gpu"}
+    detail = {"stage_ms": st_mfma, "stage_rooflines": stages, "fp32": fp32, "variants": variants or None,
+              "accuracy": acc, "proposal": prop, "roofline": roof, "cpu_baseline": cpu, "config": config,
+              "total_logl": last["total_logl"]}
+    if args.detail:
+        with open(args.detail, "w") as f:
+            json.dump(detail, f)
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": r3(value),
         "unit": "iter/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_mfma,
+        "ms_per_step": r3(ms_mfma),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "model_side": L, "rotations": S, "patterns": I,
-                   "active_pixels": Pa, "photons": d["cfg"]["photons"], "precision": "bf16x2",
-                   "insert": args.insert, "mstep": args.mstep, "interp": "trilinear", "likelihood": "true_poisson",
-                   "blur_sigma": d["cfg"]["blur"], "parallelism": "single_gpu"},
-        "fp32": fp32,
-        "speedup_vs_fp32": (fp32["ms_per_step"] / ms_mfma) if fp32 else None,
-        "stage_ms": st_mfma,
-        "total_logl": last["total_logl"],
-        "roofline": roof,
-        "stage_rooflines": stages,
+        "dtype": "bf16x2-split/fp32-acc/fp64-lse",
+        "data": "synthetic (Gaussian-atom object, Poisson patterns at random orientations; random initial model)",
+        "config": config,
+        "roofline": {k: (r3(v) if isinstance(v, float) and k != "frac" else v) for k, v in roof.items()},
         "cpu_baseline": cpu,
-        "variants": variants or None,
-        "accuracy": acc,
-        "proposal": prop,
+        "stage_ms": {k: r3(v) for k, v in st_mfma.items()},
+        "iteration_roofline_frac": r3(stages["iteration"]["frac"]),
     }
+    out["roofline"]["frac"] = round(roof["frac"], 4)
+    if fp32:
+        out["fp32"] = {"value": r3(fp32["value"]), "ms_per_step": r3(fp32["ms_per_step"]), "steps": fp32["steps"],
+                       "logl_gemm_ms": r3(fp32["stage_ms"]["logl_gemm"]),
+                       "speedup_iteration": r3(fp32["ms_per_step"] / ms_mfma),
+                       "speedup_logl_contraction": r3(fp32["stage_ms"]["logl_gemm"] / st_mfma["logl_gemm"])}
+        out["chi2_scaled"] = {"fp32_ms": r3(fp32["chi2_scaled"]["ms_per_step"]),
+                              "mfma_ms": r3(fp32["chi2_scaled_mfma"]["ms_per_step"]),
+                              "speedup": r3(fp32["chi2_scaled_mfma"]["speedup_vs_fp32"]), "steps": args.chi2_steps}
+    if acc:
+        out["accuracy"] = {p: {k: float(f"{v:.3e}") for k, v in acc[p].items()} for p in ("fp32", "bf16x2", "bf16x3")}
+    if prop is not None:
+        out["proposal"] = proposal_summary(prop)
     print(json.dumps(out))
 
 

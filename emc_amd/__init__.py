@@ -242,6 +242,17 @@ class Engine:
         self._c(lib().emc_get_resp(self._h, P))
         return P
 
+    def get_logl_block(self, s0: int, ns: int, i0: int, ni: int):
+        """Logits of rotations [s0, s0+ns) and patterns [i0, i0+ni): [ns][ni]."""
+        R = np.zeros((ns, ni), np.float32)
+        self._c(lib().emc_get_logl_block(self._h, s0, ns, i0, ni, R))
+        return R
+
+    def get_resp_block(self, s0: int, ns: int, i0: int, ni: int):
+        P = np.zeros((ns, ni), np.float32)
+        self._c(lib().emc_get_resp_block(self._h, s0, ns, i0, ni, P))
+        return P
+
     def get_lse(self):
         l = np.zeros(self.I, np.float64)
         self._c(lib().emc_get_lse(self._h, l))

@@ -100,6 +100,8 @@ SIGNATURES = {
     "emc_model_stats": (C.c_int, [ctxp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "emc_get_logl": (C.c_int, [ctxp, f32p]),
     "emc_get_resp": (C.c_int, [ctxp, f32p]),
+    "emc_get_logl_block": (C.c_int, [ctxp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f32p]),
+    "emc_get_resp_block": (C.c_int, [ctxp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, f32p]),
     "emc_get_lse": (C.c_int, [ctxp, f64p]),
     "emc_get_slice_weights": (C.c_int, [ctxp, f64p]),
     "emc_expand": (C.c_int, [ctxp, C.c_int32, C.c_int32, f32p]),

@@ -202,7 +202,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
     dfree(c->Cx); dfree(c->K2h); dfree(c->K2l); dfree(c->Aw); dfree(c->G2); dfree(c->G3); dfree(c->G4);
-    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->pdirty); dfree(c->lse); dfree(c->part); dfree(c->T); dfree(c->U); dfree(c->scal);
+    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->pdirty); dfree(c->lse); dfree(c->part); dfree(c->red); dfree(c->T); dfree(c->U); dfree(c->scal);
     dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
@@ -329,6 +329,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (st == EMC_OK) A(dalloc(&c->lse, Ip), "lse");
     c->part_len = std::max<int64_t>(emc::softmax_part_len(c->S, c->I, softmax_fused_bands(c)), 4096);
     if (st == EMC_OK) A(dalloc(&c->part, c->part_len), "part");
+    if (st == EMC_OK) A(dalloc(&c->red, kRedLen), "red");
     if (st == EMC_OK) A(dalloc(&c->T, Sp), "T");
     if (st == EMC_OK && !(c->mfma && opt.insert == EMC_INSERT_FUSED)) A(dalloc(&c->U, cp * PpMax), "U");
     if (st == EMC_OK) A(dalloc(&c->scal, SC_N), "scal");
@@ -433,11 +434,10 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
         }
     for (int i = 0; i < I; ++i) total += sumK[i];
     if (c->mfma) {
-        if (!integral) return fail(c, EMC_E_UNSUPPORTED, "set_patterns: MFMA precision modes need integer counts");
-        if (maxc > 65535.0) return fail(c, EMC_E_UNSUPPORTED, "set_patterns: counts above 65535 need precision = FP32");
-        if (c->chi2_mfma && maxc > 255.0)
-            return fail(c, EMC_E_UNSUPPORTED, "set_patterns: the chi^2 kinds on MFMA need counts <= 255 (K^2 exact in two BF16 terms)");
-        c->k_split = maxc > 256.0;
+        // Counts are one exact BF16 term up to 256.  Above that, and for non-integer patterns
+        // (read_stride-binned means, normalize_images output: emc_helper.cpp:226-236, 389-418),
+        // they are a BF16 hi + lo pair: exact for integers up to 65535, else within 2^-17 relative.
+        c->k_split = maxc > 256.0 || !integral;
         std::vector<bf16_t> hi((size_t)Ip * Pp, 0), lo, hT((size_t)Ip * Pp, 0), lT;
         if (c->k_split) {
             lo.assign((size_t)Ip * Pp, 0);
@@ -482,7 +482,8 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
         HIP_TRY(c, hipMemcpy(c->K16, hi.data(), sizeof(bf16_t) * hi.size(), hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(c->KT16, hT.data(), sizeof(bf16_t) * hT.size(), hipMemcpyHostToDevice));
         if (c->chi2_mfma) {
-            // K^2 < 2^16 as an exact BF16 pair: hi = bf16(K^2), lo = K^2 - hi (< 256, exact)
+            // K^2 as a BF16 pair hi = bf16(K^2), lo = K^2 - hi: exact for integer counts <= 255
+            // (K^2 < 2^16, lo < 256), else within 2^-17 relative
             std::vector<bf16_t> h2((size_t)Ip * Pp, 0), l2((size_t)Ip * Pp, 0);
             for (int i = 0; i < I; ++i)
                 for (int j = 0; j < Pa; ++j) {
@@ -491,7 +492,7 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
                     h2[(size_t)i * Pp + j] = h;
                     l2[(size_t)i * Pp + j] = f2bf(k2 - bf2f(h));
                 }
-            c->k2_split = maxc > 16.0;
+            c->k2_split = maxc > 16.0 || !integral;
             HIP_TRY(c, hipMemcpy(c->K2h, h2.data(), sizeof(bf16_t) * h2.size(), hipMemcpyHostToDevice));
             HIP_TRY(c, hipMemcpy(c->K2l, l2.data(), sizeof(bf16_t) * l2.size(), hipMemcpyHostToDevice));
         }
@@ -592,49 +593,71 @@ extern "C" emc_status emc_expand(emc_ctx *c, int32_t s0, int32_t count, float *o
     return EMC_OK;
 }
 
-extern "C" emc_status emc_get_logl(emc_ctx *c, float *logits)
+static bool bad_block(const emc_ctx *c, int32_t s0, int32_t ns, int32_t i0, int32_t ni)
+{
+    return s0 < 0 || ns < 0 || i0 < 0 || ni < 0 || (int64_t)s0 + ns > c->S || (int64_t)i0 + ni > c->I;
+}
+
+extern "C" emc_status emc_get_logl_block(emc_ctx *c, int32_t s0, int32_t ns, int32_t i0, int32_t ni, float *logits)
 {
     if (!c || !logits) return fail(c, EMC_E_INVALID, "get_logl: null argument");
+    if (bad_block(c, s0, ns, i0, ni)) return fail(c, EMC_E_INVALID, "get_logl: block out of range");
     if (!c->has_logl) return fail(c, EMC_E_STATE, "get_logl: logits not available (run emc_logl; the FP32 path "
                                                   "overwrites them in emc_normalize_resp)");
+    if (ns == 0 || ni == 0) return EMC_OK;
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    std::vector<double> cst(c->I);
-    HIP_TRY(c, hipMemcpy(cst.data(), c->cst, sizeof(double) * c->I, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy2D(logits, sizeof(float) * c->I, c->R, sizeof(float) * c->Ip, sizeof(float) * c->I, c->S,
-                           hipMemcpyDeviceToHost));
-    for (int s = 0; s < c->S; ++s)
-        for (int i = 0; i < c->I; ++i) {
-            float &v = logits[(size_t)s * c->I + i];
+    std::vector<double> cst(ni);
+    HIP_TRY(c, hipMemcpy(cst.data(), c->cst + i0, sizeof(double) * ni, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy2D(logits, sizeof(float) * ni, c->R + (size_t)s0 * c->Ip + i0, sizeof(float) * c->Ip,
+                           sizeof(float) * ni, ns, hipMemcpyDeviceToHost));
+    for (int s = 0; s < ns; ++s)
+        for (int i = 0; i < ni; ++i) {
+            float &v = logits[(size_t)s * ni + i];
             v = (float)((double)v + cst[i]);
         }
     return EMC_OK;
 }
 
-extern "C" emc_status emc_get_resp(emc_ctx *c, float *resp)
+extern "C" emc_status emc_get_logl(emc_ctx *c, float *logits)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "get_logl: null argument");
+    return emc_get_logl_block(c, 0, c->S, 0, c->I, logits);
+}
+
+extern "C" emc_status emc_get_resp_block(emc_ctx *c, int32_t s0, int32_t ns, int32_t i0, int32_t ni, float *resp)
 {
     if (!c || !resp) return fail(c, EMC_E_INVALID, "get_resp: null argument");
+    if (bad_block(c, s0, ns, i0, ni)) return fail(c, EMC_E_INVALID, "get_resp: block out of range");
     if (!c->has_resp) return fail(c, EMC_E_STATE, "get_resp: no responsibilities");
+    if (ns == 0 || ni == 0) return EMC_OK;
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
+    const size_t off = (size_t)s0 * c->Ip + i0;
     if (!c->mfma) {
-        HIP_TRY(c, hipMemcpy2D(resp, sizeof(float) * c->I, c->R, sizeof(float) * c->Ip, sizeof(float) * c->I, c->S,
+        HIP_TRY(c, hipMemcpy2D(resp, sizeof(float) * ni, c->R + off, sizeof(float) * c->Ip, sizeof(float) * ni, ns,
                                hipMemcpyDeviceToHost));
         return EMC_OK;
     }
-    std::vector<bf16_t> h((size_t)c->S * c->I), l((size_t)c->S * c->I);
-    HIP_TRY(c, hipMemcpy2D(h.data(), sizeof(bf16_t) * c->I, c->Ph, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * c->I,
-                           c->S, hipMemcpyDeviceToHost));
-    HIP_TRY(c, hipMemcpy2D(l.data(), sizeof(bf16_t) * c->I, c->Pl, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * c->I,
-                           c->S, hipMemcpyDeviceToHost));
+    std::vector<bf16_t> h((size_t)ns * ni), l((size_t)ns * ni);
+    HIP_TRY(c, hipMemcpy2D(h.data(), sizeof(bf16_t) * ni, c->Ph + off, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * ni,
+                           ns, hipMemcpyDeviceToHost));
+    HIP_TRY(c, hipMemcpy2D(l.data(), sizeof(bf16_t) * ni, c->Pl + off, sizeof(bf16_t) * c->Ip, sizeof(bf16_t) * ni,
+                           ns, hipMemcpyDeviceToHost));
     for (size_t k = 0; k < h.size(); ++k) resp[k] = bf2f(h[k]) + bf2f(l[k]);
     if (c->opt.scaling) {  // the pair holds the M-step's P / phi: undo the factor
-        std::vector<float> ip((size_t)c->S * c->I);
-        HIP_TRY(c, hipMemcpy2D(ip.data(), sizeof(float) * c->I, c->iphi, sizeof(float) * c->Ip, sizeof(float) * c->I,
-                               c->S, hipMemcpyDeviceToHost));
+        std::vector<float> ip((size_t)ns * ni);
+        HIP_TRY(c, hipMemcpy2D(ip.data(), sizeof(float) * ni, c->iphi + off, sizeof(float) * c->Ip, sizeof(float) * ni,
+                               ns, hipMemcpyDeviceToHost));
         for (size_t k = 0; k < ip.size(); ++k) resp[k] /= ip[k];
     }
     return EMC_OK;
+}
+
+extern "C" emc_status emc_get_resp(emc_ctx *c, float *resp)
+{
+    if (!c) return fail(c, EMC_E_INVALID, "get_resp: null argument");
+    return emc_get_resp_block(c, 0, c->S, 0, c->I, resp);
 }
 
 extern "C" emc_status emc_get_lse(emc_ctx *c, double *lse)
@@ -846,9 +869,10 @@ extern "C" emc_status emc_maximize(emc_ctx *c)
 static emc_status compress_enqueue(emc_ctx *c)
 {
     ev_start(c, EMC_STAGE_COMPRESS);
+    // divide, normalise, blur (the normaliser's positive voxels are fixed by the weighted counts)
     HIP_TRY(c, launch_divide(c, c->stream));
-    if (c->opt.blur_sigma > 0.0) HIP_TRY(c, launch_blur(c, c->stream));
     HIP_TRY(c, launch_normalize(c, true, c->stream));
+    if (c->opt.blur_sigma > 0.0) HIP_TRY(c, launch_blur(c, c->stream));
     ev_stop(c, EMC_STAGE_COMPRESS);
     return EMC_OK;
 }

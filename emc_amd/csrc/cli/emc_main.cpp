@@ -176,15 +176,9 @@ int main(int argc, char **argv)
     emc_default_options(&opt);
     opt.diff = cfg.diff;
     opt.interp = cfg.interp;
-    // the MFMA modes need integer counts (exact in BF16), and at most 255 with a chi^2 diff_type
-    // (the squared counts are an exact BF16 pair); otherwise the FP32 path
-    int32_t kmax = 0;
-    for (int32_t k : K) kmax = std::max(kmax, k);
-    const bool mfma_ok = Kf.empty() && (cfg.diff == EMC_DIFF_TRUE_POISSON || kmax <= 255);
-    opt.precision = mfma_ok ? cfg.precision : EMC_PREC_FP32;
-    if (opt.precision != cfg.precision)
-        std::fprintf(stderr, "emc: %s runs on the FP32 path\n",
-                     Kf.empty() ? "a chi^2 diff_type with counts above 255" : "non-integer patterns");
+    // every precision takes every pattern set: the MFMA modes split large or non-integer counts
+    // (read_stride-binned means, normalize_images output) into two BF16 terms
+    opt.precision = cfg.precision;
     opt.insert = cfg.insert;
     opt.sigma_start = cfg.sigma_start;
     opt.sigma_final = cfg.sigma_final;

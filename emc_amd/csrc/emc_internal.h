@@ -16,6 +16,8 @@ using bf16_t = uint16_t;  // BF16 storage (bit pattern); MFMA operands are bitca
 constexpr int kPadS = 256;
 constexpr int kPadI = 256;
 constexpr int kPadP = 128;
+// Partial sums of the model statistics (256 blocks x 4 values).
+constexpr int kRedLen = 1024;
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
@@ -179,6 +181,8 @@ struct emc_ctx {
     uint8_t *pdirty = nullptr;  // [Sp][ceil(I/256)]: 0 = that 256-pattern segment of Ph/Pl holds only zeros
     double *lse = nullptr;                             // [Ip] lse' (constants excluded)
     double *part = nullptr;                            // softmax / reduction partials
+    double *red = nullptr;                             // [kRedLen] model-statistics partials (own buffer:
+                                                       // part may hold the logL epilogue's softmax partials)
     int64_t part_len = 0;
     double *T = nullptr;                               // [Sp]
     float *U = nullptr;                                // [chunkp][Pp] (atomic insertion)

@@ -1,6 +1,9 @@
-// Compress: divide by weight (emc_cuda.cu:758-786), masked Gaussian blur (the intended math of
-// emc_cuda.cu:1404-1492) and normalisation to mean 1 over positive voxels (emc_cuda.cu:639-661,
-// 788-798), with FP64 grid-wide sums instead of the reference's single-block FP32 reduction.
+// Compress: divide by weight (emc_cuda.cu:758-786), normalisation to mean 1 over positive voxels
+// (emc_cuda.cu:639-661, 788-798; FP64 grid-wide sums instead of the reference's single-block FP32
+// reduction) and the masked Gaussian blur (the intended math of emc_cuda.cu:1404-1492), in that
+// order: the positive voxels of the divided model are those that received a non-zero weighted
+// count, a set no rounding can change, whereas the blur's ringing leaves voxels at the rounding
+// level whose sign would follow the summation order of the insertion.
 //
 // Blur: the Fourier multiplier exp(-2 pi^2 sigma^2 |d|^2 / L^2), d = min(k, L-k) per axis, is
 // separable and real-symmetric, so it equals three periodic 1-D convolutions with taps
@@ -28,7 +31,7 @@ __global__ void k_divide(const float *__restrict__ Mnum, const float *__restrict
 }
 
 // One axis of the periodic convolution.  PHASE 0 reads the model with negatives as 0; PHASE 2
-// writes the result back into the model, -1 where the model was negative (masked).
+// writes the result back into the model, where a negative (missing) voxel keeps its value.
 template <int PHASE>
 __global__ __launch_bounds__(256) void k_blur_axis(const float *__restrict__ in, float *__restrict__ out,
                                                    float *__restrict__ model, const float *__restrict__ taps, int L,
@@ -52,7 +55,10 @@ __global__ __launch_bounds__(256) void k_blur_axis(const float *__restrict__ in,
         if (PHASE == 0) a = a < 0.0f ? 0.0f : a;
         acc = __builtin_fmaf(g[d], a, acc);
     }
-    if (PHASE == 2) model[v] = model[v] < 0.0f ? -1.0f : acc;
+    if (PHASE == 2) {
+        const float m = model[v];
+        model[v] = m < 0.0f ? m : acc;
+    }
     else out[v] = acc;
 }
 
@@ -147,9 +153,10 @@ hipError_t launch_blur(const emc_ctx *c, hipStream_t st)
 
 hipError_t launch_normalize(const emc_ctx *c, bool apply, hipStream_t st)
 {
+    static_assert(4 * kPartBlocks <= kRedLen, "model statistics partials");
     hipLaunchKernelGGL(k_model_partial, dim3(kPartBlocks), dim3(256), 0, st, c->model, apply ? c->Wt : nullptr, c->N3,
-                       c->part);
-    hipLaunchKernelGGL(k_model_final, dim3(1), dim3(256), 0, st, c->part, kPartBlocks, c->scal, apply);
+                       c->red);
+    hipLaunchKernelGGL(k_model_final, dim3(1), dim3(256), 0, st, c->red, kPartBlocks, c->scal, apply);
     if (apply)
         hipLaunchKernelGGL(k_scale, dim3((unsigned)((c->N3 + 255) / 256)), dim3(256), 0, st, c->model, c->N3,
                            c->scal);

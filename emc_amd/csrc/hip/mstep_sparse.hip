@@ -37,7 +37,7 @@ static void launch_np(const emc_ctx *c, int s0, int cs, const double *gate, doub
     if (c->K8)
         hipLaunchKernelGGL((k_mstep_sparse<0, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K8,
                            nullptr, c->Pp, gate, gate_thr, c->U, c->pdirty, p_segments(c));
-    else if (c->K16lo)
+    else if (c->k_split)  // K16lo is kept allocated after a pattern set that needed it: key on k_split
         hipLaunchKernelGGL((k_mstep_sparse<2, NPASS>), grid, blk, 0, st, c->Ph, c->Pl, (int64_t)c->Ip, s0, c->I, c->K16,
                            c->K16lo, c->Pp, gate, gate_thr, c->U, c->pdirty, p_segments(c));
     else

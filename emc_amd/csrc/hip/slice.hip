@@ -262,7 +262,7 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
     double phi0 = 1.0;
     if (diff == EMC_DIFF_TRUE_POISSON) {
         if (fluence > 0.0) phi0 = fluence;
-        else if (SW > 0.0 && I > 0) phi0 = (sumK_total / I) / (SW / S);
+        else if (SW > 0.0 && I > 0 && sumK_total > 0.0) phi0 = (sumK_total / I) / (SW / S);  // else undefined: 1
     }
     const double B0 = (diff == EMC_DIFF_TRUE_POISSON && S > 0) ? phi0 * SW / S : 0.0;
     if (tid == 0) {
@@ -386,7 +386,7 @@ hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st)
 
 hipError_t launch_pattern_const(const emc_ctx *c, bool centred, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pattern_const, dim3(c->I), dim3(256), 0, st, c->K32, c->K16, c->K16lo, c->K8, c->I,
+    hipLaunchKernelGGL(k_pattern_const, dim3(c->I), dim3(256), 0, st, c->K32, c->K16, c->k_split ? c->K16lo : nullptr, c->K8, c->I,
                        c->Pp, c->colmean, c->sumK, c->scal, centred, c->cst);
     return hipGetLastError();
 }

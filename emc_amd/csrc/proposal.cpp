@@ -158,6 +158,34 @@ emc_status upload_batch(emc_ctx *c, const float *x, const float *t, int B)
     return EMC_OK;
 }
 
+// Largest packed work lists (int32 elements) and slot count any proposal iteration can need: at
+// most floor(max_fallback * I) patterns take every non-empty cell (more and the iteration
+// searches fully instead), the others at most top_k + recall cells; every count of tiles follows
+// from those slot counts per cell.  Allocated once by emc_proposal_enable, so proposal iterations
+// never allocate.
+int64_t list_bound(const emc_ctx *c, const Proposal *p, int64_t *slots_max)
+{
+    const int I = c->I, C = p->C, Pp = c->Pp;
+    int64_t cne = 0, mt_sum = 0, mt_max = 0;
+    for (int cc = 0; cc < C; ++cc) {
+        const int64_t nr = p->rbeg_h[cc + 1] - p->rbeg_h[cc];
+        if (nr == 0) continue;
+        ++cne;
+        const int64_t mt = (nr + 127) / 128;
+        mt_sum += mt;
+        mt_max = std::max(mt_max, mt);
+    }
+    const int64_t nfb = std::min<int64_t>(I, (int64_t)std::floor((double)p->o.max_fallback * I));
+    const int64_t per = std::min<int64_t>(p->o.top_k + p->o.recall, cne);
+    const int64_t slots = nfb * cne + (I - nfb) * per;
+    const int64_t n_te = mt_max * ((slots + 127) / 128 + cne);
+    const int64_t n_tm = mt_sum * (Pp / 128);
+    const int64_t n_ts = (slots + 63) / 64 + cne;
+    auto sec = [](int64_t n) { return rup(std::max<int64_t>(n, 1), 4); };
+    *slots_max = slots;
+    return sec(slots) + sec(C + 1) + sec(C) + sec(I + 1) + sec(slots) + sec(4 * n_te) + sec(4 * n_tm) + sec(4 * n_ts);
+}
+
 emc_status need_prop(emc_ctx *c, const char *what)
 {
     if (!c) return pfail(nullptr, EMC_E_INVALID, std::string(what) + ": null context");
@@ -305,14 +333,8 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
     p->off_tile_m = sec(4 * n_tm);
     p->off_tile_sm = sec(4 * n_ts);
     const int64_t need = off;
-    if (need > p->list_cap) {
-        pfree(p->lists);
-        phfree(p->lists_h);
-        const int64_t cap = need + need / 4;
-        P_HIP(c, palloc(&p->lists, cap));
-        P_HIP(c, hipHostMalloc((void **)&p->lists_h, sizeof(int32_t) * (size_t)cap, hipHostMallocDefault));
-        p->list_cap = cap;
-    }
+    if (need > p->list_cap || n_slots > p->slot_cap)  // the bound of list_bound() holds by construction
+        return pfail(c, EMC_E_STATE, "proposal: work lists exceed their preallocated bound");
     int32_t *L = p->lists_h;
     int32_t *slot_pat = L + p->off_slot_pat, *pbeg = L + p->off_pbeg, *pslot = L + p->off_pslot;
     for (int cc = 0; cc <= C; ++cc) L[p->off_sbeg + cc] = (int32_t)sbeg[cc];
@@ -357,17 +379,6 @@ emc_status proposal_build_lists(emc_ctx *c, bool *too_many)
     p->n_tiles_m = (int)n_tm;
     p->n_tiles_sm = (int)n_ts;
 
-    // device scratch sized to this iteration's lists
-    if (n_slots > p->slot_cap) {
-        pfree(p->slot_ps);
-        pfree(p->slot_pm);
-        pfree(p->slot_lse);
-        const int64_t cap = n_slots + n_slots / 4 + 64;
-        P_HIP(c, palloc(&p->slot_ps, cap));
-        P_HIP(c, palloc(&p->slot_pm, cap));
-        P_HIP(c, palloc(&p->slot_lse, cap));
-        p->slot_cap = cap;
-    }
     P_HIP(c, hipMemcpyAsync(p->lists, p->lists_h, sizeof(int32_t) * (size_t)need, hipMemcpyHostToDevice, c->stream));
     P_HIP(c, hipMemcpyAsync(p->boff, p->boff_h, sizeof(int64_t) * (size_t)C, hipMemcpyHostToDevice, c->stream));
     return EMC_OK;
@@ -602,6 +613,15 @@ static emc_status enable_impl(emc_ctx *c, const float *coarse, int32_t nc, const
     P_HIP(c, hipMemset(p->recall, 0xff, sizeof(int32_t) * (size_t)Ip * std::max(o.recall, 1)));  // -1: none yet
     P_HIP(c, palloc(&p->boff, nc));
     P_HIP(c, hipHostMalloc((void **)&p->boff_h, sizeof(int64_t) * nc, hipHostMallocDefault));
+    // grouped work lists and their slot scratch at their largest size (iterations never allocate)
+    int64_t slots_max = 0;
+    p->list_cap = list_bound(c, p, &slots_max);
+    p->slot_cap = std::max<int64_t>(slots_max, 1);
+    P_HIP(c, palloc(&p->lists, p->list_cap));
+    P_HIP(c, hipHostMalloc((void **)&p->lists_h, sizeof(int32_t) * (size_t)p->list_cap, hipHostMallocDefault));
+    P_HIP(c, palloc(&p->slot_ps, p->slot_cap));
+    P_HIP(c, palloc(&p->slot_pm, p->slot_cap));
+    P_HIP(c, palloc(&p->slot_lse, p->slot_cap));
     P_HIP(c, hipStreamSynchronize(st));
     p->rows_cap = (int)rows;
     return EMC_OK;

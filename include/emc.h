@@ -47,9 +47,9 @@ enum { EMC_INTERP_NEAREST = 0, EMC_INTERP_TRILINEAR = 1 };
  *   FP32   : direct FP32 (vector ALU), the plain path.
  *   BF16X2 : MFMA, counts exact in BF16, log W and P each split into BF16 hi+lo, FP32 accumulate.
  *   BF16X3 : as BF16X2 with log W split into three BF16 terms (hi+mid+lo).
- * Counts above 256 are not exact in BF16: the MFMA modes then also split the counts into two
- * exact BF16 terms.  The chi^2 kinds run on MFMA as expanded contractions (sum a W^2 - 2 sum a W K
- * + sum a K^2, with K^2 as two exact BF16 terms) and need counts <= 255 there. */
+ * Counts above 256 and non-integer patterns are split into two BF16 terms on the MFMA modes.
+ * The chi^2 kinds run on MFMA as expanded contractions (sum a W^2 - 2 sum a W K + sum a K^2, with
+ * K^2 as two BF16 terms, exact for integer counts <= 255). */
 enum { EMC_PREC_FP32 = 0, EMC_PREC_BF16X2 = 1, EMC_PREC_BF16X3 = 2 };
 
 /* How the M-step inserts the weighted slice sums into the model.
@@ -145,7 +145,10 @@ typedef struct emc_ctx emc_ctx;
 void emc_default_options(emc_options *opt);
 
 /* Replaces the allocation block of main.cpp:158-209 and cuda_set_device (emc_cuda.cu:866-869).
- * Allocates every device buffer for dims; no allocation happens in later calls. */
+ * Allocates every device buffer for dims.  The iteration and its stages (emc_iterate, emc_logl,
+ * emc_normalize_resp, emc_maximize, emc_compress, emc_normalize_model), with or without the
+ * proposal, allocate nothing; the input setters (the insertion lists of a new geometry, a count
+ * operand a new pattern set needs), emc_expand's staging buffer and emc_proposal_enable do. */
 emc_status emc_create(const emc_dims *dims, const emc_options *opt, emc_ctx **out);
 void emc_destroy(emc_ctx *ctx);
 const char *emc_last_error(const emc_ctx *ctx); /* ctx may be NULL: last create/host error */
@@ -162,7 +165,9 @@ emc_status emc_set_coords(emc_ctx *ctx, const float *x, const float *y, const fl
 /* cuda_allocate_mask (emc_cuda.cu:689-693).  [L*L], values 0 or 1. */
 emc_status emc_set_mask(emc_ctx *ctx, const int32_t *mask);
 /* cuda_allocate_images (emc_cuda.cu:703-709).  [I][L*L] photon counts (>= 0).  Must follow
- * emc_set_mask.  The i32 form is the exact-count path used by the MFMA modes. */
+ * emc_set_mask.  On the MFMA modes integer counts up to 256 are one exact BF16 term; larger ones
+ * and non-integer values (binned means, normalised images) are a BF16 hi + lo pair (exact for
+ * integers up to 65535, else within 2^-17 relative); the chi^2 kinds' squared counts likewise. */
 emc_status emc_set_patterns_i32(emc_ctx *ctx, const int32_t *counts);
 emc_status emc_set_patterns_f32(emc_ctx *ctx, const float *counts);
 /* cuda_allocate_model / cuda_copy_model_2_device (emc_cuda.cu:675-685).  [L^3]. */
@@ -180,6 +185,10 @@ emc_status emc_get_logl(emc_ctx *ctx, float *logits);
 /* Responsibilities [S][I] after emc_normalize_resp (the isDebug dump, main.cpp:395-417).  With
  * options.scaling on an MFMA precision the device keeps P / phi for the M-step; this returns P. */
 emc_status emc_get_resp(emc_ctx *ctx, float *resp);
+/* The same for the block of rotations [s0, s0+ns) and patterns [i0, i0+ni): out [ns][ni] (a window
+ * of the isDebug dump, for inspection at sizes where [S][I] does not fit the host). */
+emc_status emc_get_logl_block(emc_ctx *ctx, int32_t s0, int32_t ns, int32_t i0, int32_t ni, float *logits);
+emc_status emc_get_resp_block(emc_ctx *ctx, int32_t s0, int32_t ns, int32_t i0, int32_t ni, float *resp);
 /* Per-pattern log-sum-exp [I] (main.cpp:384-391, accumulated in FP64). */
 emc_status emc_get_lse(emc_ctx *ctx, double *lse);
 /* Slice sums of the last M-step: T [S] (emc_atomic.cu:114-122). */

@@ -602,8 +602,13 @@ ORC_API void orc_normalize_patterns(const float *K, int I, int P, const int32_t 
     }
 }
 
-/* Global fluence for true Poisson: mean photons per pattern over mean slice sum (both over
- * unmasked pixels; slice sum over V_s).  No per-pattern scaling. */
+/* Global fluence for true Poisson (the G8 decision; the reference's model carries no photon
+ * units): mean photons per pattern over mean slice sum, both over unmasked pixels, the slice sum
+ * over V_s (W > 0).  1 when either mean is 0 (no photon or no valid sample: undefined).  No
+ * per-pattern scaling.
+ * Pinned by: closed forms on constructed slices and patterns, the fallbacks and the scaling laws
+ * in K and W (test_oracle_fluence.py); the device's value is compared with it on every
+ * true-Poisson GPU iteration test. */
 ORC_API double orc_fluence(const double *W, const double *K, const int32_t *mask, int S, int I,
                            int P)
 {
@@ -614,7 +619,7 @@ ORC_API double orc_fluence(const double *W, const double *K, const int32_t *mask
     for (int s = 0; s < S; s++)
         for (int p = 0; p < P; p++)
             if (mask[p] != 0 && W[(size_t)s * P + p] > 0.0) sw += W[(size_t)s * P + p];
-    if (sw <= 0.0 || S == 0 || I == 0) return 1.0;
+    if (sw <= 0.0 || sk <= 0.0 || S == 0 || I == 0) return 1.0;
     return (sk / I) / (sw / S);
 }
 
@@ -746,7 +751,8 @@ ORC_API void orc_compress(double *M, const double *Mnum, const double *Wt, long 
 /* ------------------------------------------------------------------------------------------ */
 /* a15: blur -- the reference's intended masked Fourier-Gaussian (emc_cuda.cu:1404-1470):      */
 /*   mask = M >= 0; M<0 -> 0; multiply the DFT by exp(-2 pi^2 sigma^2 |d|^2 / L^2),             */
-/*   d = min(k, L-k) per axis; inverse DFT; masked voxels -> -1.                                 */
+/*   d = min(k, L-k) per axis; inverse DFT; masked voxels keep their (negative) value: -1 after  */
+/*   the divide, -1/avg once normalised (the reference's missing-voxel marker either way).        */
 /* Restated as the equivalent separable periodic convolution with taps                           */
 /*   g[d] = (1/L) sum_k exp(-2 pi^2 sigma^2 min(k,L-k)^2 / L^2) cos(2 pi k d / L).                */
 /* Pinned by: numpy FFT of the same multiplier, delta response, constants preserved (tests).    */
@@ -794,7 +800,7 @@ ORC_API void orc_blur(double *M, int L, double sigma)
         }
         double *tmp = A; A = B; B = tmp;
     }
-    for (size_t v = 0; v < N3; v++) M[v] = msk[v] ? A[v] : -1.0;
+    for (size_t v = 0; v < N3; v++) M[v] = msk[v] ? A[v] : M[v];
     free(g); free(A); free(B); free(msk);
 }
 
@@ -855,10 +861,12 @@ ORC_API double orc_iterate(double *model, int L, const float *quat, const double
     double total = orc_softmax(R, S, I, lse);
     orc_mstep(R, K, mask, S, I, P, phi, quat, X, Y, Z, L, interp, Mnum, Wt, NULL);
     free(phi);
+    /* divide, normalise (mean of the positive voxels of the divided model: a set fixed by which
+     * voxels received any weighted count, not by rounding), then the blur */
     orc_compress(model, Mnum, Wt, (long)N3);
-    if (blur_sigma > 0.0) orc_blur(model, L, blur_sigma);
     double avg = 0.0;
     orc_normalize(model, (long)N3, &avg);
+    if (blur_sigma > 0.0) orc_blur(model, L, blur_sigma);
     if (stats) { stats[0] = phi0; stats[1] = avg; }
     if (Wt_out) memcpy(Wt_out, Wt, sizeof(double) * N3);
     free(W);

@@ -56,7 +56,9 @@ def test_cli_synthetic_run(tmp_path):
     m = np.fromfile(tmp_path / "output" / "model_0001.bin", np.float32)
     w = np.fromfile(tmp_path / "output" / "weight_0001.bin", np.float32)
     assert m.size == 24 ** 3 and w.size == 24 ** 3
-    assert abs(m[m > 0].mean() - 1.0) < 1e-4   # normalised to mean 1 over positive voxels
+    # normalised to mean 1 over positive voxels, then blurred (the blur's ringing moves that mean)
+    print(f"   mean of positive voxels {m[m > 0].mean():.4f}", flush=True)
+    assert abs(m[m > 0].mean() - 1.0) < 0.05
     assert np.all(m >= 0) and np.all((m == 0) | (w > 0))  # write_model's mask (file_helper.cpp:50-65)
     assert all(l["search"] == "full" for l in lines)
 
@@ -90,13 +92,14 @@ def test_cli_reference_default_scaling_run(tmp_path):
     assert [l["iteration"] for l in lines] == [0, 1]
     assert all(np.isfinite(l["total_logl"]) for l in lines)
     m = np.fromfile(tmp_path / "output" / "model_0001.bin", np.float32)
-    assert abs(m[m > 0].mean() - 1.0) < 1e-4
+    assert abs(m[m > 0].mean() - 1.0) < 0.05  # normalised, then blurred
 
 
 @pytest.mark.gpu
 def test_cli_reference_conf_choices(tmp_path):
-    """The rest of the reference's emc.conf choices: normalize_images = true (FP32 path), the radial
-    average initial model (model_input = 3) and the responsibility dump of isDebug."""
+    """The rest of the reference's emc.conf choices: normalize_images = true (non-integer patterns,
+    on the matrix cores), the radial average initial model (model_input = 3) and the responsibility
+    dump of isDebug."""
     print("[cli] normalize_images, model_input = 3, isDebug", flush=True)
     p = tmp_path / "emc.conf"
     p.write_text(CONF + 'diff_type = "poisson";\nknown_intensity = false;\nnormalize_images = true;\n'

@@ -231,9 +231,8 @@ def test_maximize_and_compress_match_oracle(orc, small, prec, insert, interp):
         M, _ = e.get_model()
     Pin = P32.astype(np.float64)
     Mnum, Wto, T = orc.mstep(Pin, d["K"][:I].astype(np.float64), d["mask"], d["q"][:S], d["coords"], d["L"], interp)
-    Mo = orc.compress(Mnum, Wto)
+    Mo, _, _ = orc.normalize(orc.compress(Mnum, Wto))  # divide, normalise, blur
     Mo = orc.blur(Mo, d["L"], 0.5)
-    Mo, _, _ = orc.normalize(Mo)
     assert np.array_equal(Wt > 0, Wto > 0)
     assert np.allclose(Wt, Wto, rtol=1e-4, atol=1e-6 * Wto.max())
     rel = np.linalg.norm(M - Mo) / np.linalg.norm(Mo)
@@ -468,3 +467,108 @@ def test_sparse_mstep_equals_dense(orc, small, counts):
     finally:
         for e in engines.values():
             e.close()
+
+
+@pytest.mark.parametrize("mstep", ["sparse", "auto"])
+def test_reused_context_drops_the_lo_counts(small, mstep):
+    """A context that once held counts above 256 (a BF16 hi + lo pair) and then gets counts <= 256
+    with BF16 counts (count_bytes = 2) must not add the old lo terms anywhere: the sparse M-step
+    and the per-pattern constants key on the current pattern set; same U / model and lse as a
+    fresh context."""
+    log(f"[reuse] lo counts dropped, M-step {mstep}")
+    d = dict(small)
+    S, I = 600, 200
+    K = d["K"][:I]
+    assert K.max() <= 256
+    rng = np.random.default_rng(5)
+    logits = rng.normal(scale=3.0, size=(S, I))
+    Pm = np.exp(logits - logits.max(0))
+    Pm[Pm < 1e-3] = 0.0  # sparse enough for the sparse form under AUTO
+    P32 = (Pm / Pm.sum(0)).astype(np.float32)
+    out = {}
+    for reused in (False, True):
+        with engine(dict(d, K=K), "bf16x2", S=S, I=I, count_bytes=2, mstep=mstep) as e:
+            if reused:
+                e.set_patterns(K * 40)   # max > 256: the lo planes are written
+                e.logl()
+                e.normalize_resp()
+                e.set_patterns(K)        # back to one exact BF16 term
+            e.logl()
+            e.normalize_resp()
+            lse = e.get_lse()
+            e.set_resp(P32)
+            e.maximize()
+            e.compress()
+            out[reused] = (lse, e.get_model()[0])
+    assert np.array_equal(out[True][0], out[False][0])
+    rel = float(np.linalg.norm(out[True][1] - out[False][1]) / np.linalg.norm(out[False][1]))
+    log(f"   model rel L2 reused vs fresh {rel:.2e}")
+    assert rel < 1e-6  # only the float atomics' order of the insertion may differ
+
+
+def test_model_stats_between_logl_and_normalize(small):
+    """emc_model_stats between emc_logl and emc_normalize_resp leaves the logL epilogue's softmax
+    partials alone (the model reduction has its own scratch)."""
+    log("[stage order] logl -> model_stats -> normalize_resp")
+    d = small
+    res = []
+    for with_stats in (False, True):
+        with engine(d, "bf16x2", S=1000, I=300) as e:
+            e.logl()
+            if with_stats:
+                mx, avg = e.model_stats()
+                assert mx > 0 and avg > 0
+            tot = e.normalize_resp()
+            res.append((tot, e.get_lse(), e.get_resp()))
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+
+
+def _float_patterns(d, kind):
+    """Non-integer pattern sets of the reference's own configuration: normalize_images output
+    (each pattern scaled to sum P over the mask, emc_helper.cpp:389-418) and read_stride-binned
+    means (pixel sum / mask sum, emc_helper.cpp:226-236: here the mean of a 2 x 2 block of two
+    independent count images, so values are multiples of 1/4)."""
+    K = d["K"].astype(np.float32)
+    if kind == "normalized":
+        return emc_amd.normalize_patterns(K, d["mask"])
+    rng = np.random.default_rng(9)
+    extra = rng.poisson(np.maximum(K, 0.0)).astype(np.float32)
+    return ((K + extra) / 2.0 + rng.integers(0, 2, K.shape) * 0.25).astype(np.float32) * (d["mask"] != 0)
+
+
+@pytest.mark.parametrize("kind", ["normalized", "binned_means"])
+@pytest.mark.parametrize("diff,scaling", [("true_poisson", 0), ("poisson", 1)])
+def test_float_patterns_on_mfma_match_oracle(orc, small, kind, diff, scaling):
+    """Non-integer patterns run on the matrix cores as a BF16 hi + lo pair (no FP32 fallback):
+    logits and a whole iteration against the FP64 oracle, for the true Poisson likelihood and for
+    the reference's chi^2 ("poisson") with intensity scaling."""
+    log(f"[float patterns] {kind}, diff={diff}, scaling={scaling}")
+    d = dict(small)
+    Kf = _float_patterns(d, kind)
+    assert (Kf != np.floor(Kf)).any()
+    S, I = 1200, 400
+    Kf = Kf[:I]
+    sig = 0.5
+    kw = dict(diff=diff, scaling=scaling, sigma_start=sig, sigma_final=sig, blur_sigma=0.5)
+    M32 = d["m0"].astype(np.float32)
+    Kd = Kf.astype(np.float64)
+    w64 = d["w"][:S].astype(np.float64)
+    with engine(dict(d, K=Kf), "bf16x2", S=S, I=I, **kw) as e:
+        e.logl(sig)
+        R = e.get_logl()
+        st = e.iterate(0)
+        M, _ = e.get_model()
+    W = orc.slices(M32.astype(np.float64), d["L"], d["q"][:S], d["coords"], "trilinear")
+    phi = orc.scaling(W, Kd, d["mask"]) if scaling else None
+    phi0 = orc.fluence(W, Kd, d["mask"]) if diff == "true_poisson" else 1.0
+    Ro, _ = orc.logl(W, Kd, d["mask"], w64, diff, sigma=sig, phi=phi, phi0=phi0)
+    ref = orc.iterate(M32.astype(np.float64), d["L"], d["q"][:S], w64, Kd, d["mask"], d["coords"], "trilinear",
+                      diff, sigma=sig, blur_sigma=0.5, scaling=bool(scaling))
+    err = rel_logl(R, Ro)
+    rel = float(np.linalg.norm(M - ref["model"]) / np.linalg.norm(ref["model"]))
+    dl = abs(st["total_logl"] - ref["total"]) / abs(ref["total"])
+    log(f"   rel logL {err:.2e}, max abs {np.abs(R - Ro).max():.2e} nats, model rel L2 {rel:.2e}, "
+        f"total logL rel {dl:.2e}")
+    assert err < 1e-4
+    assert rel < 1e-4 and dl < 1e-5

@@ -273,7 +273,9 @@ def test_blur_equals_fft_multiplier(orc):
     ref = np.real(np.fft.ifftn(np.fft.fftn(A) * H)).ravel()
     keep = M >= 0
     assert np.allclose(out[keep], ref[keep], atol=1e-12)
-    assert np.all(out[~keep] == -1)
+    assert np.all(out[~keep] == -1)  # a missing voxel keeps its value
+    Ms = np.where(M < 0, -0.25, M)    # ... also after normalisation scaled it (-1/avg)
+    assert np.all(orc.blur(Ms, Lb, sigma)[~keep] == -0.25)
     c = orc.blur(np.full(Lb ** 3, 2.5), Lb, sigma)  # constants are preserved (h(0) = 1)
     assert np.allclose(c, 2.5)
     g = orc.blur_taps(Lb, 1e-9)  # sigma -> 0 is the identity
@@ -303,8 +305,8 @@ def test_iterate_is_the_composition_of_stages(orc):
     R, _ = orc.logl(W, K, mask, w.astype(np.float64), "true_poisson", phi0=phi0)
     Pm, lse, tot = orc.softmax(R)
     Mnum, Wt, T = orc.mstep(Pm, K, mask, q, coords, Ls, "trilinear")
-    Mc = orc.blur(orc.compress(Mnum, Wt), Ls, 0.5)
-    Mc, avg, _ = orc.normalize(Mc)
+    Mc, avg, _ = orc.normalize(orc.compress(Mnum, Wt))  # divide, normalise, then blur
+    Mc = orc.blur(Mc, Ls, 0.5)
     assert np.allclose(out["P"], Pm) and np.allclose(out["lse"], lse)
     assert np.allclose(out["model"], Mc) and np.isclose(out["total"], tot)
     assert np.isclose(out["phi0"], phi0)
