@@ -36,7 +36,8 @@ class Options(C.Structure):
                 ("sigma_start", C.c_double), ("sigma_final", C.c_double), ("sigma_half_life", C.c_int32),
                 ("device", C.c_int32), ("blur_sigma", C.c_double), ("fluence", C.c_double),
                 ("timing", C.c_int32), ("mfma_tile", C.c_int32), ("count_bytes", C.c_int32),
-                ("mstep", C.c_int32), ("scaling", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("mstep", C.c_int32), ("scaling", C.c_int32), ("tile_rows", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class IterStats(C.Structure):

@@ -47,7 +47,11 @@ hipError_t dalloc(T **p, int64_t n)
     if (n <= 0) return hipSuccess;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
     if (e != hipSuccess) return e;
-    return hipMemset(*p, 0, sizeof(T) * (size_t)n);
+    e = hipMemset(*p, 0, sizeof(T) * (size_t)n);
+    // the zero fill runs on the null stream, which the context's non-blocking stream does not wait
+    // for: finish it before any kernel on that stream can touch the buffer
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
 }
 
 template <typename T>
@@ -55,22 +59,6 @@ void dfree(T *&p)
 {
     if (p) (void)hipFree(p);
     p = nullptr;
-}
-
-// Blur taps g[d] = (1/L) sum_k exp(-2 pi^2 sigma^2 min(k,L-k)^2 / L^2) cos(2 pi k d / L).
-std::vector<float> blur_taps(int L, double sigma)
-{
-    std::vector<float> g(L);
-    for (int d = 0; d < L; ++d) {
-        double acc = 0.0;
-        for (int k = 0; k < L; ++k) {
-            const int kk = std::min(k, L - k);
-            acc += std::exp(-2.0 * M_PI * M_PI * sigma * sigma * (double)kk * kk / ((double)L * L)) *
-                   std::cos(2.0 * M_PI * (double)k * d / L);
-        }
-        g[d] = (float)(acc / L);
-    }
-    return g;
 }
 
 void ev_start(emc_ctx *c, int stage)
@@ -199,11 +187,12 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->K32); dfree(c->KT32); dfree(c->K16); dfree(c->K16lo); dfree(c->KT16); dfree(c->KT16lo); dfree(c->sumK);
     dfree(c->K8); dfree(c->KT8);
     dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
-    dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
+    dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colp); dfree(c->colpart); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
     dfree(c->Cx); dfree(c->K2h); dfree(c->K2l); dfree(c->Aw); dfree(c->G2); dfree(c->G3); dfree(c->G4);
     dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->pdirty); dfree(c->lse); dfree(c->part); dfree(c->red); dfree(c->T); dfree(c->U); dfree(c->scal);
-    dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vox); dfree(c->bin_frac); dfree(c->bin_units);
+    dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vbeg); dfree(c->bin_frac); dfree(c->bin_units); dfree(c->bin_blocks); dfree(c->bin_groups);
+    dfree(c->bin_gbeg); dfree(c->bin_part); dfree(c->bin_gpart);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -283,6 +272,8 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK) A(dalloc(&c->pdirty, Sp * (int64_t)p_segments(c)), "pdirty");
         if (st == EMC_OK) A(hipMemset(c->pdirty, 1, sizeof(uint8_t) * Sp * p_segments(c)), "pdirty");
         if (st == EMC_OK) A(dalloc(&c->colsum, 2 * (int64_t)PpMax), "colsum");
+        if (st == EMC_OK) A(dalloc(&c->colp, colp_len(c->S, PpMax)), "colp");
+        if (st == EMC_OK) A(dalloc(&c->colpart, colpart_len(PpMax)), "colpart");
         if (st == EMC_OK) A(dalloc(&c->lw32, Sp * PpMax), "lw32");
         if (c->chi2_mfma) {
             const int64_t nops = (opt.scaling ? 4 : 2) * (int64_t)c->la_terms;
@@ -334,9 +325,10 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (st == EMC_OK && !(c->mfma && opt.insert == EMC_INSERT_FUSED)) A(dalloc(&c->U, cp * PpMax), "U");
     if (st == EMC_OK) A(dalloc(&c->scal, SC_N), "scal");
     if (st == EMC_OK) A(hipHostMalloc((void **)&c->scal_h, sizeof(double) * SC_N, hipHostMallocDefault), "scal_h");
+    if (st == EMC_OK) A(hipDeviceSynchronize(), "sync");  // every null-stream fill is done
     if (st == EMC_OK && opt.blur_sigma > 0.0) {
-        std::vector<float> g = blur_taps(c->L, opt.blur_sigma);
-        A(hipMemcpy(c->taps, g.data(), sizeof(float) * c->L, hipMemcpyHostToDevice), "taps");
+        A(launch_blur_taps(c->L, opt.blur_sigma, c->taps, c->stream), "taps");
+        if (st == EMC_OK) A(hipStreamSynchronize(c->stream), "taps");
     }
     if (st != EMC_OK) {
         std::string msg = c->err;
@@ -369,7 +361,7 @@ extern "C" emc_status emc_set_rotations(emc_ctx *c, const float *q, const float 
     (void)hipFree(wd);
     HIP_TRY(c, e);
     c->has_rot = true;
-    c->has_logl = c->has_resp = false;
+    c->has_logl = c->has_resp = c->has_lse = false;
     EMC_TRY(emc_proposal_disable(c));  // its cell partition belonged to the old rotations
     return refresh_bins(c);
 }
@@ -391,7 +383,7 @@ extern "C" emc_status emc_set_coords(emc_ctx *c, const float *x, const float *y,
         HIP_TRY(c, hipMemcpy(m.data(), c->mask, sizeof(int32_t) * c->P, hipMemcpyDeviceToHost));
         EMC_TRY(rebuild_active(c, h.data(), m.data()));
     }
-    c->has_logl = c->has_resp = false;
+    c->has_logl = c->has_resp = c->has_lse = false;
     return refresh_bins(c);
 }
 
@@ -410,7 +402,7 @@ extern "C" emc_status emc_set_mask(emc_ctx *c, const int32_t *mask)
     EMC_TRY(rebuild_active(c, c->has_coords ? h.data() : nullptr, mask));
     c->has_mask = true;
     c->has_K = false;  // compaction changed
-    c->has_logl = c->has_resp = false;
+    c->has_logl = c->has_resp = c->has_lse = false;
     EMC_TRY(emc_proposal_disable(c));  // its pattern features used the old compaction
     return refresh_bins(c);
 }
@@ -459,8 +451,8 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
             HIP_TRY(c, dalloc(&c->K16lo, (int64_t)Ip * round_up(P, kPadP)));
             HIP_TRY(c, dalloc(&c->KT16lo, (int64_t)Ip * round_up(P, kPadP)));
         }
-        // one byte per count when exact (<= 255) for the 256-row kernels
-        const bool u8 = c->opt.count_bytes == 0 && maxc <= 255.0;
+        // one byte per count when exact (integers <= 255) for the 256-row kernels
+        const bool u8 = c->opt.count_bytes == 0 && integral && maxc <= 255.0;
         if (u8) {
             std::vector<uint8_t> b((size_t)Ip * Pp, 0), bT((size_t)Ip * Pp, 0);
             for (int i = 0; i < I; ++i)
@@ -516,7 +508,7 @@ static emc_status set_patterns_impl(emc_ctx *c, const float *Kf)
     c->sumK_total = total;
     c->max_count = (int32_t)std::min(maxc, 2147483647.0);
     c->has_K = true;
-    c->has_logl = c->has_resp = false;
+    c->has_logl = c->has_resp = c->has_lse = false;
     return EMC_OK;
 }
 
@@ -549,7 +541,7 @@ extern "C" emc_status emc_set_model(emc_ctx *c, const float *model)
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, hipMemcpy(c->model, model, sizeof(float) * c->N3, hipMemcpyHostToDevice));
     c->has_model = true;
-    c->has_logl = c->has_resp = false;
+    c->has_logl = c->has_resp = c->has_lse = false;
     return EMC_OK;
 }
 
@@ -570,7 +562,7 @@ extern "C" emc_status emc_model_stats(emc_ctx *c, double *mx, double *avg)
     if (!c) return fail(c, EMC_E_INVALID, "model_stats: null context");
     if (!c->has_model) return fail(c, EMC_E_STATE, "model_stats: no model");
     HIP_TRY(c, hipSetDevice(c->opt.device));
-    HIP_TRY(c, launch_normalize(c, false, c->stream));
+    HIP_TRY(c, launch_normalize(c, false, false, c->stream));
     EMC_TRY(sync_scalars(c));
     if (mx) *mx = c->scal_h[SC_MAX];
     if (avg) *avg = c->scal_h[SC_AVG];
@@ -663,7 +655,7 @@ extern "C" emc_status emc_get_resp(emc_ctx *c, float *resp)
 extern "C" emc_status emc_get_lse(emc_ctx *c, double *lse)
 {
     if (!c || !lse) return fail(c, EMC_E_INVALID, "get_lse: null argument");
-    if (!c->has_resp) return fail(c, EMC_E_STATE, "get_lse: no responsibilities");
+    if (!c->has_lse) return fail(c, EMC_E_STATE, "get_lse: no E-step since the inputs changed");
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     std::vector<double> cst(c->I);
@@ -676,7 +668,7 @@ extern "C" emc_status emc_get_lse(emc_ctx *c, double *lse)
 extern "C" emc_status emc_get_slice_weights(emc_ctx *c, double *T)
 {
     if (!c || !T) return fail(c, EMC_E_INVALID, "get_slice_weights: null argument");
-    if (!c->has_resp) return fail(c, EMC_E_STATE, "get_slice_weights: no responsibilities");
+    if (!c->has_resp && !c->has_lse) return fail(c, EMC_E_STATE, "get_slice_weights: no responsibilities");
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     HIP_TRY(c, hipMemcpy(T, c->T, sizeof(double) * c->S, hipMemcpyDeviceToHost));
@@ -738,7 +730,7 @@ extern "C" emc_status emc_logl(emc_ctx *c, float sigma)
     EMC_TRY(logl_enqueue(c, sigma));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->has_logl = true;
-    c->has_resp = false;
+    c->has_resp = c->has_lse = false;
     return EMC_OK;
 }
 
@@ -768,7 +760,7 @@ extern "C" emc_status emc_normalize_resp(emc_ctx *c, double *total_logl)
     HIP_TRY(c, hipSetDevice(c->opt.device));
     EMC_TRY(softmax_enqueue(c));
     EMC_TRY(sync_scalars(c));
-    c->has_resp = true;
+    c->has_resp = c->has_lse = true;
     if (!c->mfma) c->has_logl = false;  // P overwrote the logits
     if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "normalize_resp: non-finite log-sum-exp");
     if (total_logl) EMC_TRY(total_logl_host(c, total_logl));
@@ -814,10 +806,23 @@ extern "C" emc_status emc_set_resp(emc_ctx *c, const float *resp)
                                hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy2D(c->Pl, sizeof(bf16_t) * c->Ip, l.data(), sizeof(bf16_t) * I, sizeof(bf16_t) * I, S,
                                hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemset(c->pdirty, 1, sizeof(uint8_t) * c->Sp * p_segments(c)));
+        HIP_TRY(c, hipMemsetAsync(c->pdirty, 1, sizeof(uint8_t) * c->Sp * p_segments(c), c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
     }
     HIP_TRY(c, hipMemcpy(c->T, T.data(), sizeof(double) * c->Sp, hipMemcpyHostToDevice));
     c->has_resp = true;
+    return EMC_OK;
+}
+
+extern "C" emc_status emc_set_mstep_sums(emc_ctx *c, const float *Mnum, const float *Wt)
+{
+    if (!c || !Mnum || !Wt) return fail(c, EMC_E_INVALID, "set_mstep_sums: null argument");
+    for (int64_t v = 0; v < c->N3; ++v)
+        if (!std::isfinite(Mnum[v]) || !std::isfinite(Wt[v])) return fail(c, EMC_E_INVALID, "set_mstep_sums: non-finite value");
+    HIP_TRY(c, hipSetDevice(c->opt.device));
+    HIP_TRY(c, hipMemcpy(c->Mnum, Mnum, sizeof(float) * c->N3, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(c->Wt, Wt, sizeof(float) * c->N3, hipMemcpyHostToDevice));
+    c->has_mstep = true;
     return EMC_OK;
 }
 
@@ -869,10 +874,10 @@ extern "C" emc_status emc_maximize(emc_ctx *c)
 static emc_status compress_enqueue(emc_ctx *c)
 {
     ev_start(c, EMC_STAGE_COMPRESS);
-    // divide, normalise, blur (the normaliser's positive voxels are fixed by the weighted counts)
-    HIP_TRY(c, launch_divide(c, c->stream));
-    HIP_TRY(c, launch_normalize(c, true, c->stream));
-    if (c->opt.blur_sigma > 0.0) HIP_TRY(c, launch_blur(c, c->stream));
+    // divide, normalise over the voxels that received weight, blur (the numerator, free after the
+    // divide, is the blur's second buffer)
+    HIP_TRY(c, launch_compress_buffers(c->L, c->opt.blur_sigma > 0.0, c->Mnum, c->Wt, c->model, c->taps, c->tmp3,
+                                       c->Mnum, c->red, c->scal, c->stream));
     ev_stop(c, EMC_STAGE_COMPRESS);
     return EMC_OK;
 }
@@ -886,8 +891,8 @@ extern "C" emc_status emc_compress(emc_ctx *c, double *model_avg)
     EMC_TRY(compress_enqueue(c));
     EMC_TRY(sync_scalars(c));
     if (model_avg) *model_avg = c->scal_h[SC_AVG];
-    c->has_logl = c->has_resp = false;
-    if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "compress: no positive voxel to normalise");
+    c->has_logl = c->has_resp = c->has_lse = false;
+    if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "compress: the model is zero wherever it has weight");
     return EMC_OK;
 }
 
@@ -897,10 +902,10 @@ extern "C" emc_status emc_normalize_model(emc_ctx *c, double *model_avg)
     if (!c->has_model) return fail(c, EMC_E_STATE, "normalize_model: no model");
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
-    HIP_TRY(c, launch_normalize(c, true, c->stream));
+    HIP_TRY(c, launch_normalize(c, true, false, c->stream));
     EMC_TRY(sync_scalars(c));
     if (model_avg) *model_avg = c->scal_h[SC_AVG];
-    c->has_logl = c->has_resp = false;
+    c->has_logl = c->has_resp = c->has_lse = false;
     if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "normalize_model: no positive voxel");
     return EMC_OK;
 }
@@ -1031,6 +1036,7 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
     if (pr) ++pr->iter;
     c->has_logl = full && c->mfma;
     c->has_resp = full;  // a proposal iteration leaves per-cell blocks, not the [S][I] matrix
+    c->has_lse = true;   // lse and the slice sums T of either search
     c->has_mstep = true;
     if (c->scal_h[SC_NONFINITE] != 0.0) return fail(c, EMC_E_NONFINITE, "iterate: non-finite log-sum-exp or model normaliser");
     if (stats) {

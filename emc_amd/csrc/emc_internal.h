@@ -128,6 +128,7 @@ struct emc_ctx {
 
     bool has_rot = false, has_coords = false, has_mask = false, has_K = false, has_model = false;
     bool has_logl = false, has_resp = false, has_mstep = false;
+    bool has_lse = false;  // lse and T of the last E-step (full or grouped) are current
 
     hipStream_t stream = nullptr;
     hipEvent_t ev[2 * EMC_N_STAGES] = {};  // start/stop per stage
@@ -154,6 +155,8 @@ struct emc_ctx {
     double *sumW = nullptr;   // [Sp]
     int32_t *ns = nullptr;    // [Sp]
     double *colsum = nullptr; // [Pp][2] per pixel: sum over rotations of log W, valid count
+    float2 *colp = nullptr;   // [ceil(S / 16)][Pp] per statistics block: (sum of log W, valid count)
+    double2 *colpart = nullptr; // [32][Pp] fixed-order partial column sums
     float *colmean = nullptr; // [Pp]
     float *bias = nullptr;    // [Sp] per-rotation additive term of the logits
     float *rscale = nullptr;  // [Sp] chi^2 factor 1/(2 n_s sigma^2)
@@ -188,13 +191,20 @@ struct emc_ctx {
     float *U = nullptr;                                // [chunkp][Pp] (atomic insertion)
     double *scal = nullptr;                            // [SC_N] device scalars
     double *scal_h = nullptr;                          // pinned host mirror
-    // binned insertion: (s, j) pair indices sorted by (chunk, model brick); work units
-    uint32_t *bin_vals = nullptr;                      // [S * Pa] pair index s * Pa + j, (s, j) order per unit
-    uint32_t *bin_perm = nullptr;                      // [S * Pa] positions in bin_vals, base-voxel order per unit
-    uint16_t *bin_vox = nullptr;                       // [S * Pa] base voxel in the unit's halo brick (perm order)
-    float *bin_frac = nullptr;                         // [S * Pa][3] trilinear fractions (perm order)
-    int32_t *bin_units = nullptr;                      // [n_units][3] = key, begin, end
-    std::vector<int> bin_chunk_units;                  // unit range of each rotation chunk
+    // binned insertion: valid (s, j) pairs sorted by (chunk, model brick), cut into units of at most
+    // 2,048 pairs; per unit a base-voxel permutation; partial halo bricks per block (<= 4 units of a
+    // brick, one workgroup) and per group of blocks
+    uint32_t *bin_vals = nullptr;                      // [S * Pa] packed s << jbits | j, (s, j) order per brick
+    uint16_t *bin_perm = nullptr;                      // [S * Pa] each pair's position in base-voxel order in its unit
+    uint16_t *bin_vbeg = nullptr;                      // [n_units][520] start of each base voxel's run
+    float *bin_frac = nullptr;                         // [n_units][3][2048] trilinear fractions (base-voxel order)
+    int32_t *bin_units = nullptr;                      // [n_units][4] = key, begin, end, 0
+    int32_t *bin_blocks = nullptr;                     // [n_blocks][4] = key, first unit, end unit, 0 (one workgroup)
+    int32_t *bin_groups = nullptr;                     // [n_groups][4] = key, first block, end block (per chunk), 0
+    int32_t *bin_gbeg = nullptr;                       // [chunks][bricks + 1] first group of each brick
+    float *bin_part = nullptr;                         // [blocks of a chunk][2][9^3] block partials
+    float *bin_gpart = nullptr;                        // [groups of a chunk][2][9^3] group partials
+    std::vector<int> bin_chunk_blocks, bin_chunk_groups; // block / group range of each rotation chunk
     bool bins_valid = false;
     // learned proposal (null when disabled)
     emc::Proposal *prop = nullptr;
@@ -209,6 +219,9 @@ hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t st);
 hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st);
 
 // Slicing statistics over rotations [0, S): sumW, ns and (if colsum) sum of log W per pixel.
+// colp / colpart hold colp_len / colpart_len elements.
+int64_t colp_len(int S, int Pp);
+int64_t colpart_len(int Pp);
 hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st);
 // Fluence, per-rotation bias / chi^2 scale, column means.
 hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st);
@@ -268,10 +281,17 @@ hipError_t launch_insert(const emc_ctx *c, int s0, int cs, hipStream_t st);
 // Brick-binned insertion: build the sorted pair lists (setup; allocates), then per chunk.
 hipError_t build_bins(emc_ctx *c, std::string &err);
 hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st);
-// Compress: divide, blur, normalise.
-hipError_t launch_divide(const emc_ctx *c, hipStream_t st);
-hipError_t launch_blur(const emc_ctx *c, hipStream_t st);
-hipError_t launch_normalize(const emc_ctx *c, bool apply, hipStream_t st);
+// Model statistics (max, sum of Wt) and the normaliser: the mean of the model over the voxels
+// with weight (support: Wt > 0, after the divide) or over the positive voxels; apply scales every
+// voxel by its inverse.
+hipError_t launch_normalize(const emc_ctx *c, bool apply, bool support, hipStream_t st);
+// Blur taps of side L (FP64 sums, stored as float).
+hipError_t launch_blur_taps(int L, double sigma, float *taps, hipStream_t st);
+// Compress on explicit buffers: divide, normalise over the voxels with weight (FP64 partials in
+// red [kRedLen], scalars in scal [SC_N]), blur when asked (t1, t2: two L^3 volumes; t2 may be
+// Mnum, which the divide has consumed).
+hipError_t launch_compress_buffers(int L, bool blur, const float *Mnum, const float *Wt, float *model,
+                                   const float *taps, float *t1, float *t2, double *red, double *scal, hipStream_t st);
 // Deterministic single-workgroup sum of n doubles into *out.
 hipError_t launch_sum_f64(const double *x, int n, double *out, hipStream_t st);
 

@@ -1,9 +1,14 @@
-// Compress: divide by weight (emc_cuda.cu:758-786), normalisation to mean 1 over positive voxels
-// (emc_cuda.cu:639-661, 788-798; FP64 grid-wide sums instead of the reference's single-block FP32
-// reduction) and the masked Gaussian blur (the intended math of emc_cuda.cu:1404-1492), in that
-// order: the positive voxels of the divided model are those that received a non-zero weighted
-// count, a set no rounding can change, whereas the blur's ringing leaves voxels at the rounding
-// level whose sign would follow the summation order of the insertion.
+// Compress: divide by weight (emc_cuda.cu:758-786), normalisation to mean 1 (emc_cuda.cu:639-661,
+// 788-798; FP64 grid-wide sums instead of the reference's single-block FP32 reduction) and the
+// masked Gaussian blur (the intended math of emc_cuda.cu:1404-1492), in that order.
+//
+// The reference averages the model over its positive voxels.  Which voxels are positive after the
+// divide depends on where the responsibilities underflow (a voxel whose every contribution is a
+// P below the float range is 0 in FP32 and positive in FP64: 6 % of the voxels at 32^3), and after
+// the blur on the sign of rounding-level ringing.  Here the mean is taken over the voxels that
+// received weight (Wt > 0, zeros included) of the divided model: the same set in every precision
+// and at every summation order, and the same mean wherever the support holds no zero.  The
+// initial model (no weights yet) is averaged over its positive voxels as in the reference.
 //
 // Blur: the Fourier multiplier exp(-2 pi^2 sigma^2 |d|^2 / L^2), d = min(k, L-k) per axis, is
 // separable and real-symmetric, so it equals three periodic 1-D convolutions with taps
@@ -62,7 +67,8 @@ __global__ __launch_bounds__(256) void k_blur_axis(const float *__restrict__ in,
     else out[v] = acc;
 }
 
-// Per-block FP64 sum and count of positive voxels and the max (emc_cuda.cu:629-661).
+// Per-block FP64 sum and count over the averaged voxels (those with Wt > 0 when Wt is given, else
+// the positive ones, emc_cuda.cu:639-661), the max (emc_cuda.cu:629-637) and the sum of Wt.
 __global__ __launch_bounds__(256) void k_model_partial(const float *__restrict__ M, const float *__restrict__ Wt,
                                                        int64_t n, double *__restrict__ part)
 {
@@ -70,9 +76,10 @@ __global__ __launch_bounds__(256) void k_model_partial(const float *__restrict__
     double s = 0.0, cnt = 0.0, mx = -1e300, ws = 0.0;
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
         const float x = M[v];
-        if (x > 0.0f) { s += x; cnt += 1.0; }
+        const float w = Wt ? Wt[v] : 0.0f;
+        if (Wt ? w > 0.0f : x > 0.0f) { s += x; cnt += 1.0; }
         mx = fmax(mx, (double)x);
-        if (Wt) ws += Wt[v];
+        ws += w;
     }
     s = wave_sum(s);
     cnt = wave_sum(cnt);
@@ -131,32 +138,54 @@ __global__ void k_scale(float *__restrict__ M, int64_t n, const double *__restri
 
 constexpr int kPartBlocks = 256;
 
+// Blur taps g[d] = (1/L) sum_k exp(-2 pi^2 sigma^2 min(k, L-k)^2 / L^2) cos(2 pi k d / L), FP64.
+__global__ void k_blur_taps(int L, double sigma, float *__restrict__ taps)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= L) return;
+    double acc = 0.0;
+    for (int k = 0; k < L; ++k) {
+        const int kk = min(k, L - k);
+        acc += exp(-2.0 * M_PI * M_PI * sigma * sigma * (double)kk * kk / ((double)L * L)) *
+               cos(2.0 * M_PI * (double)k * d / L);
+    }
+    taps[d] = (float)(acc / L);
+}
+
 }  // namespace
 
-hipError_t launch_divide(const emc_ctx *c, hipStream_t st)
+hipError_t launch_blur_taps(int L, double sigma, float *taps, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_divide, dim3((unsigned)((c->N3 + 255) / 256)), dim3(256), 0, st, c->Mnum, c->Wt, c->model,
-                       c->N3);
+    hipLaunchKernelGGL(k_blur_taps, dim3((L + 63) / 64), dim3(64), 0, st, L, sigma, taps);
     return hipGetLastError();
 }
 
-hipError_t launch_blur(const emc_ctx *c, hipStream_t st)
+// The steps of one compress on explicit buffers: divide, normalise over the voxels with weight,
+// blur (t1 and t2: two L^3 volumes of scratch).
+hipError_t launch_compress_buffers(int L, bool blur, const float *Mnum, const float *Wt, float *model,
+                                   const float *taps, float *t1, float *t2, double *red, double *scal, hipStream_t st)
 {
-    const dim3 grid((unsigned)((c->N3 + 255) / 256)), blk(256);
-    const size_t sh = sizeof(float) * c->L;
-    // tmp3 holds the first two passes; Mnum (free after divide) is the second buffer.
-    hipLaunchKernelGGL(k_blur_axis<0>, grid, blk, sh, st, c->model, c->tmp3, nullptr, c->taps, c->L, 0);
-    hipLaunchKernelGGL(k_blur_axis<1>, grid, blk, sh, st, c->tmp3, c->Mnum, nullptr, c->taps, c->L, 1);
-    hipLaunchKernelGGL(k_blur_axis<2>, grid, blk, sh, st, c->Mnum, nullptr, c->model, c->taps, c->L, 2);
+    const int64_t N3 = (int64_t)L * L * L;
+    const dim3 grid((unsigned)((N3 + 255) / 256)), blk(256);
+    hipLaunchKernelGGL(k_divide, grid, blk, 0, st, Mnum, Wt, model, N3);
+    hipLaunchKernelGGL(k_model_partial, dim3(kPartBlocks), blk, 0, st, model, Wt, N3, red);
+    hipLaunchKernelGGL(k_model_final, dim3(1), blk, 0, st, red, kPartBlocks, scal, true);
+    hipLaunchKernelGGL(k_scale, grid, blk, 0, st, model, N3, scal);
+    if (blur) {
+        const size_t sh = sizeof(float) * L;
+        hipLaunchKernelGGL(k_blur_axis<0>, grid, blk, sh, st, model, t1, nullptr, taps, L, 0);
+        hipLaunchKernelGGL(k_blur_axis<1>, grid, blk, sh, st, t1, t2, nullptr, taps, L, 1);
+        hipLaunchKernelGGL(k_blur_axis<2>, grid, blk, sh, st, t2, nullptr, model, taps, L, 2);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_normalize(const emc_ctx *c, bool apply, hipStream_t st)
+hipError_t launch_normalize(const emc_ctx *c, bool apply, bool support, hipStream_t st)
 {
     static_assert(4 * kPartBlocks <= kRedLen, "model statistics partials");
-    hipLaunchKernelGGL(k_model_partial, dim3(kPartBlocks), dim3(256), 0, st, c->model, apply ? c->Wt : nullptr, c->N3,
-                       c->red);
-    hipLaunchKernelGGL(k_model_final, dim3(1), dim3(256), 0, st, c->red, kPartBlocks, c->scal, apply);
+    hipLaunchKernelGGL(k_model_partial, dim3(kPartBlocks), dim3(256), 0, st, c->model, support ? c->Wt : nullptr,
+                       c->N3, c->red);
+    hipLaunchKernelGGL(k_model_final, dim3(1), dim3(256), 0, st, c->red, kPartBlocks, c->scal, support);
     if (apply)
         hipLaunchKernelGGL(k_scale, dim3((unsigned)((c->N3 + 255) / 256)), dim3(256), 0, st, c->model, c->N3,
                            c->scal);

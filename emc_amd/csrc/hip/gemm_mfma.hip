@@ -14,6 +14,8 @@
 // rows padded to 72 BF16 (144 B) so the 16-byte fragment reads of 16 lanes hit 64 distinct
 // banks; the next k-tile is fetched into registers while the current one is multiplied.
 // The fused-scatter epilogue stages the accumulators through LDS, 64 rows at a time.
+#include <algorithm>
+
 #include "../emc_internal.h"
 #include "geom.h"
 
@@ -50,6 +52,7 @@ struct EpiArgs {
     const double *gate;
     double gate_thr;
     bool accumulate;  // store epilogue: C += acc instead of C = acc
+    int tile_rows;    // large-tile kernel: row panels per tile group (0 = kTileRows)
 };
 
 __device__ __forceinline__ void scatter_one(const EpiArgs &e, const float *m, float Ts, int j, float u)
@@ -241,6 +244,7 @@ hipError_t run(dim3 grid, hipStream_t st, const bf16_t *A0, const bf16_t *A1, co
 // Workgroups are renumbered so that each XCD (round-robin dispatch: workgroup b runs on XCD b % 8)
 // works on a contiguous band of tiles, which share A row panels in that XCD's L2.
 constexpr int BM2 = 256, BK2 = 32, LDK2 = BK2 + 8;
+constexpr int kTileRows = 1;  // default tile grouping of the large-tile kernel
 
 // Counts stored as bytes (exact to 255) widen to BF16 on the way to LDS: 8 bytes -> 8 BF16.
 __device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
@@ -260,7 +264,7 @@ template <int NA, int EPI, int BN2, bool BU8>
 __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
                                                     const bf16_t *__restrict__ A2, int64_t lda,
                                                     const void *__restrict__ Bv, int64_t ldb, int Kd, int ntn,
-                                                    int n_base, EpiArgs e)
+                                                    int n_base, int gm, EpiArgs e)
 {
     if (e.gate && *e.gate <= e.gate_thr) return;
     constexpr int TN = BN2 / 64;                       // 32-wide tiles per wave along N
@@ -268,11 +272,15 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
     constexpr int STAGE = (NA * BM2 + BN2) * LDK2;     // BF16 elements per stage
     __shared__ __attribute__((aligned(16))) bf16_t smem[2 * STAGE];
 
-    // XCD-aware renumbering (bijective for any grid size)
+    // XCD-aware renumbering (bijective for any grid size), then tiles in groups of gm row panels:
+    // consecutive tiles t walk the gm rows of a group fastest, so the workgroups an XCD runs at
+    // once share gm A panels and (their count / gm) B panels in its L2
     const int nwg = gridDim.x, b = blockIdx.x;
     const int base = nwg / 8, rem = nwg % 8, xcd = b % 8, loc = b / 8;
     const int t = xcd * base + min(xcd, rem) + loc;
-    const int m0 = (t / ntn) * BM2, n0 = n_base + (t % ntn) * BN2;
+    const int ntm = nwg / ntn, gi = t / (gm * ntn), gr = t % (gm * ntn);
+    const int rows = min(gm, ntm - gi * gm);
+    const int m0 = (gi * gm + gr % rows) * BM2, n0 = n_base + (gr / rows) * BN2;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN2 / 2);
@@ -457,8 +465,9 @@ template <int NA, int EPI, int BN2, bool BU8>
 hipError_t run2(int ntm, int ntn, int n_base, hipStream_t st, const bf16_t *A0, const bf16_t *A1, const bf16_t *A2,
                 int64_t lda, const void *B, int64_t ldb, int Kd, const EpiArgs &e)
 {
+    const int gm = std::max(1, std::min(e.tile_rows > 0 ? e.tile_rows : kTileRows, ntm));
     hipLaunchKernelGGL((k_gemm_mfma2<NA, EPI, BN2, BU8>), dim3(ntm * ntn), dim3(512), 0, st, A0, A1, A2, lda, B, ldb,
-                       Kd, ntn, n_base, e);
+                       Kd, ntn, n_base, gm, e);
     return hipGetLastError();
 }
 
@@ -501,6 +510,7 @@ int softmax_fused_bands(const emc_ctx *c) { return (int)((int64_t)((c->S + c->ch
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
 {
     EpiArgs e{};
+    e.tile_rows = c->opt.tile_rows;
     e.C = c->R + (size_t)s0 * c->Ip;
     e.ldc = c->Ip;
     e.m_valid = cs;
@@ -532,6 +542,7 @@ hipError_t launch_gemm_store_mfma(const emc_ctx *c, int na, const bf16_t *A0, co
 {
     if (na < 1 || na > 3 || c->chunkp % BM2 || c->Ip % 128 || c->Pp % BK2) return hipErrorInvalidValue;
     EpiArgs e{};
+    e.tile_rows = c->opt.tile_rows;
     e.C = C;
     e.ldc = c->Ip;
     e.m_valid = c->chunkp;
@@ -544,6 +555,7 @@ hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const
                              hipStream_t st)
 {
     EpiArgs e{};
+    e.tile_rows = c->opt.tile_rows;
     e.gate = fused ? nullptr : gate;
     e.gate_thr = gate_thr;
     e.C = c->U;

@@ -46,8 +46,9 @@ __global__ void k_fill(T *__restrict__ p, T v, int64_t n)
         p[i] = v;
 }
 
-// Per-rotation slice sum over V_s = {active j : W > 0}, |V_s|, and per pixel the sum of log W and
-// the number of rotations with a valid sample (colsum[2j], colsum[2j+1]).
+// Per-rotation slice sum over V_s = {active j : W > 0}, |V_s|, and per pixel the workgroup's sum of
+// log W and number of rotations with a valid sample (colp[block][j] = (sum, count); summed over
+// the blocks in a fixed order by k_colsum_*, so the column sums are deterministic).
 // kStatsG rotations per workgroup; each thread walks pixels j = tid, tid+256, ... and keeps the
 // kStatsG partial sums in registers.  With lw, the pass also keeps log W ([Sp][Pp], -inf where
 // W <= 0) for the operand split, which then needs no second gather from the model.
@@ -56,7 +57,7 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
                                                      const float *__restrict__ rotm, int S,
                                                      const float *__restrict__ axyz, int Pa, int Pp,
                                                      double *__restrict__ sumW, int32_t *__restrict__ ns,
-                                                     double *__restrict__ colsum, float *__restrict__ lw)
+                                                     float2 *__restrict__ colp, float *__restrict__ lw)
 {
     __shared__ float rm[kStatsG][9];
     __shared__ double red_w[4][kStatsG];
@@ -92,10 +93,7 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
                 if (lw) lw[(size_t)(s0 + g) * Pp + j] = l;
             }
         }
-        if (colsum) {
-            atomicAdd(colsum + 2 * j, (double)col);
-            atomicAdd(colsum + 2 * j + 1, (double)colv);
-        }
+        if (colp) colp[(size_t)blockIdx.x * Pp + j] = make_float2(col, (float)colv);
     }
     const int wave = tid >> 6, lane = tid & 63;
 #pragma unroll
@@ -109,6 +107,51 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
         sumW[s0 + tid] = red_w[0][tid] + red_w[1][tid] + red_w[2][tid] + red_w[3][tid];
         ns[s0 + tid] = red_n[0][tid] + red_n[1][tid] + red_n[2][tid] + red_n[3][tid];
     }
+}
+
+// Column sums of the statistics pass, in two fixed-order steps: part r of the blocks
+// [r * per, (r + 1) * per) per pixel, then the kColParts parts in order.
+constexpr int kColParts = 32;
+
+__global__ __launch_bounds__(256) void k_colsum_part(const float2 *__restrict__ colp, int nblk, int Pa, int Pp,
+                                                     double2 *__restrict__ cpart)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= Pa) return;
+    const int r = blockIdx.y, per = (nblk + kColParts - 1) / kColParts;
+    const int b0 = r * per, b1 = min(nblk, b0 + per);
+    double a = 0.0, n = 0.0;
+    int b = b0;
+    for (; b + 4 <= b1; b += 4) {  // four independent loads, added in order
+        const float2 v0 = colp[(size_t)b * Pp + j], v1 = colp[(size_t)(b + 1) * Pp + j];
+        const float2 v2 = colp[(size_t)(b + 2) * Pp + j], v3 = colp[(size_t)(b + 3) * Pp + j];
+        a += v0.x; n += v0.y;
+        a += v1.x; n += v1.y;
+        a += v2.x; n += v2.y;
+        a += v3.x; n += v3.y;
+    }
+    for (; b < b1; ++b) {
+        const float2 v = colp[(size_t)b * Pp + j];
+        a += v.x;
+        n += v.y;
+    }
+    cpart[(size_t)r * Pp + j] = make_double2(a, n);
+}
+
+__global__ __launch_bounds__(256) void k_colsum_final(const double2 *__restrict__ cpart, int Pa, int Pp,
+                                                      double *__restrict__ colsum)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= Pp) return;
+    double a = 0.0, n = 0.0;
+    if (j < Pa)
+        for (int r = 0; r < kColParts; ++r) {
+            const double2 v = cpart[(size_t)r * Pp + j];
+            a += v.x;
+            n += v.y;
+        }
+    colsum[2 * j] = a;
+    colsum[2 * j + 1] = n;
 }
 
 // FP32 slice rows for the plain contraction.  KIND: TRUE_POISSON -> A0 = log(phi0 W) (0 if
@@ -359,20 +402,25 @@ hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+int64_t colp_len(int S, int Pp) { return (int64_t)((S + kStatsG - 1) / kStatsG) * Pp; }
+int64_t colpart_len(int Pp) { return (int64_t)kColParts * Pp; }
+
 hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st)
 {
-    if (want_colsum) {
-        hipError_t e = launch_fill_f64(c->colsum, 0.0, 2 * (int64_t)c->Pp, st);
-        if (e != hipSuccess) return e;
-    }
-    const dim3 grid((c->S + kStatsG - 1) / kStatsG);
-    double *cs = want_colsum ? c->colsum : nullptr;
+    const int nblk = (c->S + kStatsG - 1) / kStatsG;
+    float2 *cp = want_colsum ? c->colp : nullptr;
     if (c->opt.interp == EMC_INTERP_NEAREST)
-        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_NEAREST>, grid, dim3(256), 0, st, c->model, c->L,
-                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs, c->lw32);
+        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_NEAREST>, dim3(nblk), dim3(256), 0, st, c->model, c->L,
+                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cp, c->lw32);
     else
-        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_TRILINEAR>, grid, dim3(256), 0, st, c->model, c->L,
-                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cs, c->lw32);
+        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_TRILINEAR>, dim3(nblk), dim3(256), 0, st, c->model, c->L,
+                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cp, c->lw32);
+    if (want_colsum) {
+        hipLaunchKernelGGL(k_colsum_part, dim3((c->Pa + 255) / 256, kColParts), dim3(256), 0, st, c->colp, nblk, c->Pa,
+                           c->Pp, c->colpart);
+        hipLaunchKernelGGL(k_colsum_final, dim3((c->Pp + 255) / 256), dim3(256), 0, st, c->colpart, c->Pa, c->Pp,
+                           c->colsum);
+    }
     return hipGetLastError();
 }
 

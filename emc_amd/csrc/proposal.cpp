@@ -47,7 +47,11 @@ hipError_t palloc(T **p, int64_t n)
     if (n <= 0) return hipSuccess;
     hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
     if (e != hipSuccess) return e;
-    return hipMemset(*p, 0, sizeof(T) * (size_t)n);
+    e = hipMemset(*p, 0, sizeof(T) * (size_t)n);
+    // the zero fill runs on the null stream, which the context's non-blocking stream does not wait
+    // for: finish it before any kernel on that stream can touch the buffer
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    return e;
 }
 
 template <typename T>
@@ -610,7 +614,7 @@ static emc_status enable_impl(emc_ctx *c, const float *coarse, int32_t nc, const
     P_HIP(c, palloc(&p->pmass, Ip));
     P_HIP(c, palloc(&p->cap, Ip));
     P_HIP(c, palloc(&p->recall, (int64_t)Ip * std::max(o.recall, 1)));
-    P_HIP(c, hipMemset(p->recall, 0xff, sizeof(int32_t) * (size_t)Ip * std::max(o.recall, 1)));  // -1: none yet
+    P_HIP(c, hipMemsetAsync(p->recall, 0xff, sizeof(int32_t) * (size_t)Ip * std::max(o.recall, 1), st));  // -1: none yet
     P_HIP(c, palloc(&p->boff, nc));
     P_HIP(c, hipHostMalloc((void **)&p->boff_h, sizeof(int64_t) * nc, hipHostMallocDefault));
     // grouped work lists and their slot scratch at their largest size (iterations never allocate)
@@ -773,8 +777,8 @@ extern "C" emc_status emc_mlp_set_params(emc_ctx *c, const float *W1, const floa
         if (!std::isfinite(v)) return pfail(c, EMC_E_INVALID, "mlp_set_params: non-finite parameter");
     P_HIP(c, hipStreamSynchronize(c->stream));
     P_HIP(c, hipMemcpy(p->par, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice));
-    P_HIP(c, hipMemset(p->am, 0, sizeof(float) * p->n_par));
-    P_HIP(c, hipMemset(p->av, 0, sizeof(float) * p->n_par));
+    P_HIP(c, hipMemsetAsync(p->am, 0, sizeof(float) * p->n_par, c->stream));
+    P_HIP(c, hipMemsetAsync(p->av, 0, sizeof(float) * p->n_par, c->stream));
     P_HIP(c, mlp_to_bf16(p->par, p->n_par, p->par16, c->stream));
     P_HIP(c, hipStreamSynchronize(c->stream));
     p->adam_t = 0;

@@ -98,7 +98,9 @@ typedef struct emc_options {
                                 and the M-step sums K P / phi (emc_atomic.cu:106-108); both sums
                                 of phi are contractions over pixels.  chi^2 kinds only (any
                                 precision).  0 = phi = 1 (known_intensity = true) */
-    int32_t reserved[3];
+    int32_t tile_rows;       /* large-tile MFMA contractions: row panels per group of consecutive tiles
+                                (L2 reuse within an XCD); 0 = the built-in default */
+    int32_t reserved[2];
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone
@@ -124,7 +126,8 @@ typedef struct emc_iter_stats {
     double total_logl; /* sum_i lse_i, FP64 (the convergence measure) */
     double sigma;      /* annealed width used (main.cpp:251) */
     double fluence;    /* global fluence used (TRUE_POISSON) */
-    double model_avg;  /* mean of positive voxels before normalisation (emc_cuda.cu:639-661) */
+    double model_avg;  /* normaliser: mean of the divided model over the voxels with weight (Wt > 0;
+                          the reference averages its positive voxels, emc_cuda.cu:639-661) */
     double weight_sum; /* sum of the insertion weights Wt */
     float stage_ms[EMC_N_STAGES]; /* HIP-event times when options.timing = 1 */
     /* learned proposal (zero / -1 when it is not enabled) */
@@ -189,7 +192,7 @@ emc_status emc_get_resp(emc_ctx *ctx, float *resp);
  * of the isDebug dump, for inspection at sizes where [S][I] does not fit the host). */
 emc_status emc_get_logl_block(emc_ctx *ctx, int32_t s0, int32_t ns, int32_t i0, int32_t ni, float *logits);
 emc_status emc_get_resp_block(emc_ctx *ctx, int32_t s0, int32_t ns, int32_t i0, int32_t ni, float *resp);
-/* Per-pattern log-sum-exp [I] (main.cpp:384-391, accumulated in FP64). */
+/* Per-pattern log-sum-exp [I] of the last E-step, full or proposal (main.cpp:384-391, FP64). */
 emc_status emc_get_lse(emc_ctx *ctx, double *lse);
 /* Slice sums of the last M-step: T [S] (emc_atomic.cu:114-122). */
 emc_status emc_get_slice_weights(emc_ctx *ctx, double *T);
@@ -211,9 +214,13 @@ emc_status emc_normalize_resp(emc_ctx *ctx, double *total_logl);
 emc_status emc_set_resp(emc_ctx *ctx, const float *resp);
 /* cuda_reset_model x2 + chunk loop of cuda_update_slices (main.cpp:443-462). */
 emc_status emc_maximize(emc_ctx *ctx);
-/* cuda_divide_model_by_weight + model blur + cuda_normalize_model (main.cpp:477-478). */
+/* cuda_divide_model_by_weight + cuda_normalize_model + model blur (main.cpp:477-478).  The model is
+ * divided by its mean over the voxels that received weight (Wt > 0), which, unlike the reference's
+ * mean over positive voxels, does not depend on where responsibilities underflow; the blur then
+ * runs on the normalised model (negative = missing voxels keep their value). */
 emc_status emc_compress(emc_ctx *ctx, double *model_avg);
-/* cuda_normalize_model (emc_cuda.cu:788-798) on the current model (main.cpp:180). */
+/* cuda_normalize_model (emc_cuda.cu:788-798) on the current model (main.cpp:180): mean over the
+ * positive voxels, as the reference (an initial model has no weights). */
 emc_status emc_normalize_model(emc_ctx *ctx, double *model_avg);
 /* One EMC iteration, main.cpp:239-480: sigma schedule, logl, normalise, maximize, compress.
  * stats may be NULL.  Synchronises before returning. */

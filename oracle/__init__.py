@@ -80,6 +80,8 @@ def lib():
         L.orc_blur.restype = None
         L.orc_normalize.argtypes = [f64p, C.c_long, C.POINTER(C.c_double)]
         L.orc_normalize.restype = C.c_long
+        L.orc_normalize_support.argtypes = [f64p, f64p, C.c_long, C.POINTER(C.c_double)]
+        L.orc_normalize_support.restype = C.c_long
         L.orc_sigma.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int]
         L.orc_sigma.restype = C.c_double
         L.orc_iterate.argtypes = [f64p, C.c_int, f32p, f64p, C.c_int, f64p, i32p, C.c_int, f32p, f32p,
@@ -271,6 +273,14 @@ def normalize(M):
     out = np.array(M, np.float64, copy=True).ravel()
     avg = C.c_double(0.0)
     cnt = lib().orc_normalize(out, out.size, C.byref(avg))
+    return out, avg.value, cnt
+
+
+def normalize_support(M, Wt):
+    """The iteration's normaliser: M / mean(M over Wt > 0), on every voxel."""
+    out = np.array(M, np.float64, copy=True).ravel()
+    avg = C.c_double(0.0)
+    cnt = lib().orc_normalize_support(out, np.ascontiguousarray(Wt, np.float64).ravel(), out.size, C.byref(avg))
     return out, avg.value, cnt
 
 

@@ -822,6 +822,24 @@ ORC_API long orc_normalize(double *M, long n, double *avg)
     return cnt;
 }
 
+/* a14 as the iteration applies it after the divide: avg = the mean of M over the voxels that
+ * received weight (Wt > 0, zeros included) -- the set the reference's positive voxels would be if
+ * no responsibility underflowed; M *= 1/avg on every voxel.  Returns the voxel count (0: none, no
+ * scaling).  Pinned by closed forms and by its equality with orc_normalize when no supported voxel
+ * is zero (test_oracle_stages.py). */
+ORC_API long orc_normalize_support(double *M, const double *Wt, long n, double *avg)
+{
+    double sum = 0.0;
+    long cnt = 0;
+    for (long v = 0; v < n; v++)
+        if (Wt[v] > 0.0) { sum += M[v]; cnt++; }
+    *avg = cnt ? sum / (double)cnt : 0.0;
+    if (cnt == 0 || !(*avg > 0.0)) return 0;
+    const double inv = 1.0 / *avg;
+    for (long v = 0; v < n; v++) M[v] *= inv;
+    return cnt;
+}
+
 /* a6: annealed width (main.cpp:251) */
 ORC_API double orc_sigma(int iteration, double s_start, double s_final, int half_life)
 {
@@ -861,11 +879,10 @@ ORC_API double orc_iterate(double *model, int L, const float *quat, const double
     double total = orc_softmax(R, S, I, lse);
     orc_mstep(R, K, mask, S, I, P, phi, quat, X, Y, Z, L, interp, Mnum, Wt, NULL);
     free(phi);
-    /* divide, normalise (mean of the positive voxels of the divided model: a set fixed by which
-     * voxels received any weighted count, not by rounding), then the blur */
+    /* divide, normalise over the voxels with weight, then the blur */
     orc_compress(model, Mnum, Wt, (long)N3);
     double avg = 0.0;
-    orc_normalize(model, (long)N3, &avg);
+    orc_normalize_support(model, Wt, (long)N3, &avg);
     if (blur_sigma > 0.0) orc_blur(model, L, blur_sigma);
     if (stats) { stats[0] = phi0; stats[1] = avg; }
     if (Wt_out) memcpy(Wt_out, Wt, sizeof(double) * N3);

@@ -56,9 +56,9 @@ def test_cli_synthetic_run(tmp_path):
     m = np.fromfile(tmp_path / "output" / "model_0001.bin", np.float32)
     w = np.fromfile(tmp_path / "output" / "weight_0001.bin", np.float32)
     assert m.size == 24 ** 3 and w.size == 24 ** 3
-    # normalised to mean 1 over positive voxels, then blurred (the blur's ringing moves that mean)
-    print(f"   mean of positive voxels {m[m > 0].mean():.4f}", flush=True)
-    assert abs(m[m > 0].mean() - 1.0) < 0.05
+    # normalised to mean 1 over the voxels with weight, then blurred (which moves that mean a little)
+    print(f"   mean over the voxels with weight {m[w > 0].mean():.4f}", flush=True)
+    assert abs(m[w > 0].mean() - 1.0) < 0.1
     assert np.all(m >= 0) and np.all((m == 0) | (w > 0))  # write_model's mask (file_helper.cpp:50-65)
     assert all(l["search"] == "full" for l in lines)
 
@@ -92,7 +92,8 @@ def test_cli_reference_default_scaling_run(tmp_path):
     assert [l["iteration"] for l in lines] == [0, 1]
     assert all(np.isfinite(l["total_logl"]) for l in lines)
     m = np.fromfile(tmp_path / "output" / "model_0001.bin", np.float32)
-    assert abs(m[m > 0].mean() - 1.0) < 0.05  # normalised, then blurred
+    w = np.fromfile(tmp_path / "output" / "weight_0001.bin", np.float32)
+    assert abs(m[w > 0].mean() - 1.0) < 0.1  # normalised, then blurred
 
 
 @pytest.mark.gpu

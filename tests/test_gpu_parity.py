@@ -6,7 +6,6 @@ import pytest
 
 import emc_amd
 from emc_amd import workloads as wl
-from conftest import rel_up_to_normaliser
 
 pytestmark = pytest.mark.gpu
 
@@ -231,7 +230,7 @@ def test_maximize_and_compress_match_oracle(orc, small, prec, insert, interp):
         M, _ = e.get_model()
     Pin = P32.astype(np.float64)
     Mnum, Wto, T = orc.mstep(Pin, d["K"][:I].astype(np.float64), d["mask"], d["q"][:S], d["coords"], d["L"], interp)
-    Mo, _, _ = orc.normalize(orc.compress(Mnum, Wto))  # divide, normalise, blur
+    Mo, _, _ = orc.normalize_support(orc.compress(Mnum, Wto), Wto)  # divide, normalise, blur
     Mo = orc.blur(Mo, d["L"], 0.5)
     assert np.array_equal(Wt > 0, Wto > 0)
     assert np.allclose(Wt, Wto, rtol=1e-4, atol=1e-6 * Wto.max())
@@ -457,12 +456,11 @@ def test_sparse_mstep_equals_dense(orc, small, counts):
                 st = e.iterate(it)
                 out[mode] = e.get_model()[0], st
             for mode in ("sparse", "auto"):
-                rel, dscale = rel_up_to_normaliser(out[mode][0], out["dense"][0])
-                npos = int((out["dense"][0] > 0).sum())
+                Ma, Mb = out[mode][0].astype(np.float64), out["dense"][0].astype(np.float64)
+                rel = float(np.linalg.norm(Ma - Mb) / np.linalg.norm(Mb))
                 dw = abs(out[mode][1]["weight_sum"] - out["dense"][1]["weight_sum"]) / out["dense"][1]["weight_sum"]
-                log(f"   it {it} {mode}: model rel L2 vs dense {rel:.2e} (scale {dscale:.2e}), weight sum rel {dw:.2e}")
+                log(f"   it {it} {mode}: model rel L2 vs dense {rel:.2e}, weight sum rel {dw:.2e}")
                 assert rel < 1e-6 and dw < 1e-6
-                assert dscale < 2.5 / npos  # at most two voxels change sign
             m = out["dense"][0]
     finally:
         for e in engines.values():
@@ -572,3 +570,22 @@ def test_float_patterns_on_mfma_match_oracle(orc, small, kind, diff, scaling):
         f"total logL rel {dl:.2e}")
     assert err < 1e-4
     assert rel < 1e-4 and dl < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x2"])
+def test_repeated_runs_are_bit_identical(small, prec):
+    """Binned insertion sums in a fixed order (no atomics), the pixel column sums of the slicing
+    statistics too, and every other reduction of the iteration has a fixed order: two runs of the
+    same iterations give bit-identical models, responsibilities and log-likelihoods."""
+    log(f"[determinism] {prec}, 3 iterations twice")
+    d = small
+    outs = []
+    for run in range(2):
+        with engine(d, prec, insert="binned", blur_sigma=0.5) as e:
+            tots = [e.iterate(it)["total_logl"] for it in range(3)]
+            M, Wt = e.get_model()
+            outs.append((tots, M, Wt, e.get_lse(), e.get_resp()))
+    a, b = outs
+    assert a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)

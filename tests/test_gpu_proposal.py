@@ -6,7 +6,6 @@ import pytest
 
 import emc_amd
 from emc_amd import workloads as wl
-from conftest import rel_up_to_normaliser
 
 pytestmark = pytest.mark.gpu
 
@@ -132,13 +131,10 @@ def test_grouped_iteration_with_every_cell_equals_full_search(d, mstep):
         assert sa1["n_fallback"] == 0 and abs(sa1["work_fraction"] - 1.0) < 1e-12
         ma, wa = a.get_model()
         mb, wb = b.get_model()
-    # up to the global normaliser: a voxel at the rounding level can change sign with the last bit
-    # of P, which moves the positive-voxel count and the model's scale by 1/count
-    rel, dscale = rel_up_to_normaliser(ma, mb)
-    npos = int((mb > 0).sum())
+    rel = float(np.linalg.norm(ma.astype(np.float64) - mb) / np.linalg.norm(mb))
     dl = abs(sa1["total_logl"] - sb1["total_logl"]) / abs(sb1["total_logl"])
-    log(f"  model rel L2 {rel:.2e} (scale {dscale:.2e}, 1/npos {1.0 / npos:.2e}), total logL rel {dl:.2e}")
-    assert rel < 1e-6 and dscale < 1.5 / npos
+    log(f"  model rel L2 {rel:.2e}, total logL rel {dl:.2e}")
+    assert rel < 1e-6
     assert dl < 1e-6  # the per-band FP32 exp of the online softmax differs with the band layout
     assert abs(sa1["weight_sum"] - sb1["weight_sum"]) / sb1["weight_sum"] < 1e-6
 

@@ -290,6 +290,22 @@ def test_normalize_mean_of_positive_is_one(orc):
     assert np.isclose(out[out > 0].mean(), 1.0)
 
 
+def test_normalize_over_the_weighted_support(orc):
+    """The iteration's normaliser: the mean over the voxels with weight, zeros included; equal to
+    the reference's mean over positive voxels when no supported voxel is zero; missing voxels
+    (no weight, -1) are scaled but not averaged."""
+    M = np.array([2.0, 4.0, -1.0, 0.0, 6.0, 0.0])
+    Wt = np.array([1.0, 0.5, 0.0, 2.0, 3.0, 0.0])
+    out, avg, cnt = orc.normalize_support(M, Wt)
+    assert cnt == 4 and avg == 3.0                   # (2 + 4 + 0 + 6) / 4
+    assert np.allclose(out, M / 3.0)
+    M2 = np.array([2.0, 4.0, -1.0, 6.0])
+    W2 = np.array([1.0, 1.0, 0.0, 1.0])
+    assert np.allclose(orc.normalize_support(M2, W2)[0], orc.normalize(M2)[0])
+    z, a0, c0 = orc.normalize_support(np.zeros(3), np.ones(3))  # nothing to normalise: unchanged
+    assert c0 == 0 and a0 == 0.0 and np.all(z == 0.0)
+
+
 def test_iterate_is_the_composition_of_stages(orc):
     rng = np.random.default_rng(11)
     Ls = 12
@@ -305,7 +321,7 @@ def test_iterate_is_the_composition_of_stages(orc):
     R, _ = orc.logl(W, K, mask, w.astype(np.float64), "true_poisson", phi0=phi0)
     Pm, lse, tot = orc.softmax(R)
     Mnum, Wt, T = orc.mstep(Pm, K, mask, q, coords, Ls, "trilinear")
-    Mc, avg, _ = orc.normalize(orc.compress(Mnum, Wt))  # divide, normalise, then blur
+    Mc, avg, _ = orc.normalize_support(orc.compress(Mnum, Wt), Wt)  # divide, normalise, then blur
     Mc = orc.blur(Mc, Ls, 0.5)
     assert np.allclose(out["P"], Pm) and np.allclose(out["lse"], lse)
     assert np.allclose(out["model"], Mc) and np.isclose(out["total"], tot)
@@ -327,6 +343,6 @@ def test_iterate_with_scaling_is_the_composition_of_stages(orc):
     R, _ = orc.logl(W, K, mask, w.astype(np.float64), "poisson", sigma=0.4, phi=phi)
     Pm, lse, tot = orc.softmax(R)
     Mnum, Wt, T = orc.mstep(Pm, K, mask, q, coords, Ls, "trilinear", phi=phi)
-    Mc, avg, _ = orc.normalize(orc.compress(Mnum, Wt))
+    Mc, avg, _ = orc.normalize_support(orc.compress(Mnum, Wt), Wt)
     assert np.allclose(out["P"], Pm) and np.allclose(out["lse"], lse)
     assert np.allclose(out["model"], Mc) and np.isclose(out["total"], tot)
