@@ -275,6 +275,11 @@ class Engine:
     def set_resp(self, P):
         self._c(lib().emc_set_resp(self._h, np.ascontiguousarray(P, np.float32).reshape(-1)))
 
+    def set_mstep_sums(self, numerator, weight):
+        """Replace the M-step sums (model numerator and weight, [L^3]) that compress divides."""
+        self._c(lib().emc_set_mstep_sums(self._h, np.ascontiguousarray(numerator, np.float32).reshape(-1),
+                                         np.ascontiguousarray(weight, np.float32).reshape(-1)))
+
     def maximize(self):
         self._c(lib().emc_maximize(self._h))
 

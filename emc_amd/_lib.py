@@ -109,6 +109,7 @@ SIGNATURES = {
     "emc_logl": (C.c_int, [ctxp, C.c_float]),
     "emc_normalize_resp": (C.c_int, [ctxp, C.POINTER(C.c_double)]),
     "emc_set_resp": (C.c_int, [ctxp, f32p]),
+    "emc_set_mstep_sums": (C.c_int, [ctxp, f32p, f32p]),
     "emc_maximize": (C.c_int, [ctxp]),
     "emc_compress": (C.c_int, [ctxp, C.POINTER(C.c_double)]),
     "emc_normalize_model": (C.c_int, [ctxp, C.POINTER(C.c_double)]),

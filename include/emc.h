@@ -212,6 +212,10 @@ emc_status emc_normalize_resp(emc_ctx *ctx, double *total_logl);
  * options.scaling the M-step then uses P / phi with the phi of the last likelihood pass (1 before
  * any). */
 emc_status emc_set_resp(emc_ctx *ctx, const float *resp);
+/* Replaces the M-step sums (model numerator and weight, [L^3] each) for stage tests and external
+ * M-steps, the state emc_compress starts from (the reference's d_model_updated and d_weight after
+ * the insertion loop, main.cpp:443-462). */
+emc_status emc_set_mstep_sums(emc_ctx *ctx, const float *numerator, const float *weight);
 /* cuda_reset_model x2 + chunk loop of cuda_update_slices (main.cpp:443-462). */
 emc_status emc_maximize(emc_ctx *ctx);
 /* cuda_divide_model_by_weight + cuda_normalize_model + model blur (main.cpp:477-478).  The model is

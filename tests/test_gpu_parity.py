@@ -589,3 +589,27 @@ def test_repeated_runs_are_bit_identical(small, prec):
     assert a[0] == b[0]
     for x, y in zip(a[1:], b[1:]):
         assert np.array_equal(x, y)
+
+
+def test_compress_of_given_sums_matches_oracle(orc):
+    """emc_set_mstep_sums + emc_compress: divide, normalise over the voxels with weight, blur, on
+    M-step sums with empty (zero-weight) voxels and zero numerators, against the oracle."""
+    log("[compress] given M-step sums")
+    rng = np.random.default_rng(17)
+    L = 24
+    Wt = rng.uniform(0.0, 3.0, L ** 3).astype(np.float32)
+    Wt[rng.random(L ** 3) < 0.2] = 0.0
+    Mnum = (rng.uniform(0.0, 2.0, L ** 3) * Wt).astype(np.float32)
+    Mnum[rng.random(L ** 3) < 0.1] = 0.0
+    with emc_amd.Engine(L, 4, 4, blur_sigma=0.5) as e:
+        e.set_mstep_sums(Mnum, Wt)
+        avg = e.compress()
+        M, W = e.get_model()
+    Mo, avgo, _ = orc.normalize_support(orc.compress(Mnum.astype(np.float64), Wt.astype(np.float64)),
+                                        Wt.astype(np.float64))
+    Mo = orc.blur(Mo, L, 0.5)
+    assert np.array_equal(W, Wt)
+    assert abs(avg - avgo) <= 1e-6 * avgo
+    miss = Wt == 0
+    assert np.allclose(M[miss], -1.0 / avgo, rtol=1e-6) and np.allclose(Mo[miss], -1.0 / avgo)
+    assert float(np.linalg.norm(M - Mo) / np.linalg.norm(Mo)) < 1e-6
