@@ -114,6 +114,7 @@ SIGNATURES = {
     "emc_compress": (C.c_int, [ctxp, C.POINTER(C.c_double)]),
     "emc_normalize_model": (C.c_int, [ctxp, C.POINTER(C.c_double)]),
     "emc_iterate": (C.c_int, [ctxp, C.c_int32, C.POINTER(IterStats)]),
+    "emc_compress_device": (C.c_int, [C.c_int32, C.c_double, vp, vp, vp, vp, vp]),
     "emc_rotation_count": (C.c_int32, [C.c_int32]),
     "emc_generate_rotations": (C.c_int, [C.c_int32, f32p, f32p]),
     "emc_coords_ewald": (C.c_int, [C.c_int32, C.c_float, C.c_float, C.c_float, f32p, f32p, f32p]),

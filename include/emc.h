@@ -230,6 +230,18 @@ emc_status emc_normalize_model(emc_ctx *ctx, double *model_avg);
  * stats may be NULL.  Synchronises before returning. */
 emc_status emc_iterate(emc_ctx *ctx, int32_t iteration, emc_iter_stats *stats);
 
+/* ---- allocation-free entry point -------------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  emc_compress's divide,
+ * normalisation (mean over the voxels with weight) and blur, on caller-owned device buffers of one
+ * L^3 model on the current device: model = blur(numerator / weight / avg), -1/avg where weight is 0;
+ * *avg (device pointer, may be NULL) receives the normaliser.  stream is a hipStream_t (NULL: the
+ * default stream); the call only enqueues.  Allocates nothing: its scratch (two L^3 volumes for
+ * L <= 128, about 16 MiB) is a static per-device array that every call on that device shares, so
+ * calls that overlap on one device race.  L outside 4..128 is EMC_E_INVALID. */
+emc_status emc_compress_device(int32_t L, double blur_sigma, const float *numerator, const float *weight,
+                               float *model, double *avg, void *stream);
+
 /* ---- host utilities (no GPU needed) -------------------------------------------------------- */
 
 /* n_to_samples(n)/2 (rotations.c:43, :457). */
