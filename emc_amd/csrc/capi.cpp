@@ -97,11 +97,21 @@ emc_status sync_scalars(emc_ctx *c)
 
 int n_chunks(const emc_ctx *c) { return (c->S + c->chunk - 1) / c->chunk; }
 
+// Active (mask != 0) pixels numbered j = 0..Pa-1 in 8 x 8 detector tiles (tiles row-major, pixels
+// row-major inside a tile): the 64 lanes of a wave that walk consecutive j then sample a compact
+// patch of the rotated detector, whose trilinear corners share far fewer cache lines than a
+// 64-pixel row's (slicing), and a model brick's pixels of one rotation form longer runs of j
+// (insertion).  Every kernel reads the detector position through pix[j] or the coordinates.
 emc_status rebuild_active(emc_ctx *c, const float *coords_h, const int32_t *mask_h)
 {
+    constexpr int kTile = 8;
     std::vector<int32_t> pix;
-    for (int p = 0; p < c->P; ++p)
-        if (mask_h[p] != 0) pix.push_back(p);
+    const int L = c->L;
+    for (int ty = 0; ty < L; ty += kTile)
+        for (int tx = 0; tx < L; tx += kTile)
+            for (int y = ty; y < std::min(L, ty + kTile); ++y)
+                for (int x = tx; x < std::min(L, tx + kTile); ++x)
+                    if (mask_h[y * L + x] != 0) pix.push_back(y * L + x);
     const int Pa = (int)pix.size();
     if (Pa == 0) return fail(c, EMC_E_INVALID, "mask has no active pixel");
     const int Pp = (int)round_up(Pa, kPadP);
