@@ -260,6 +260,84 @@ __device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
     return o;
 }
 
+// logL epilogue of the 256-row large-tile kernels (8 waves: 4 along M x 2 along N, each a 64-row
+// by BN2/2-column block of 2 x TN 32x32 tiles).  Accumulator layout: lane holds column lane % 32
+// of each 32x32 tile and rows 8*(r/4) + 4*(lane/32) + r%4, r = 0..15.  Stores R = acc + bias_s,
+// then (when e.pm is set) the softmax partials of this 256-row band, restating k_sm_partial's
+// online max / sum of exp with the same -1e10 floors: column maxima over the lane pair, then over
+// the 4 row-waves through LDS; then sums of exp(v - max) the same way.  lds: >= 12 BN2 bytes,
+// free (the main loop's stages).
+template <int TN, int BN2>
+__device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm,
+                                              int wn, void *lds)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float cmax[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) cmax[j] = -1.0e10f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m >= e.m_valid) continue;
+            const float add = e.bias[m];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const int n = n0 + wn + j * 32 + (lane & 31);
+                const float v = acc[i][j][r] + add;
+                if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = v;
+                cmax[j] = fmaxf(cmax[j], v);
+            }
+        }
+    if (!e.pm) return;
+    float *red = reinterpret_cast<float *>(lds);            // [4][BN2] maxima
+    double *reds = reinterpret_cast<double *>(lds) + 2 * BN2;  // [4][BN2] sums (after the maxima)
+    const int wrow = wave >> 1;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        cmax[j] = fmaxf(cmax[j], __shfl_xor(cmax[j], 32, 64));
+        if (lane < 32) red[wrow * BN2 + wn + j * 32 + lane] = cmax[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        const int cl = wn + j * 32 + (lane & 31);
+        cmax[j] = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
+    }
+    double csum[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) csum[j] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            if (m >= e.m_valid) continue;
+            const float add = e.bias[m];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+                const float d = (acc[i][j][r] + add) - cmax[j];
+                if (d > -1.0e10f) csum[j] += (double)expf(d);
+            }
+        }
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+        csum[j] += __shfl_xor(csum[j], 32, 64);
+        if (lane < 32) reds[wrow * BN2 + wn + j * 32 + lane] = csum[j];
+    }
+    __syncthreads();
+    for (int cl = tid; cl < BN2; cl += 512) {
+        const int n = n0 + cl;
+        if (n >= e.n_valid) continue;
+        const float mx = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
+        const double sm = reds[cl] + reds[BN2 + cl] + reds[2 * BN2 + cl] + reds[3 * BN2 + cl];
+        const int64_t o = (e.band0 + m0 / BM2) * e.ldp + n;
+        e.pm[o] = mx;
+        e.ps[o] = sm;
+    }
+}
+
 template <int NA, int EPI, int BN2, bool BU8>
 __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
                                                     const bf16_t *__restrict__ A2, int64_t lda,
@@ -386,79 +464,170 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
         __syncthreads();  // the epilogue reuses the LDS
     }
 
-    // accumulator layout: lane holds column lane % 32 of each 32x32 tile and rows
-    // 8*(r/4) + 4*(lane/32) + r%4, r = 0..15; the stored value is acc + bias (logL) or acc
-    float cmax[TN];
+    if constexpr (EPI == EPI_LOGL) {
+        logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem);
+    } else {
 #pragma unroll
-    for (int j = 0; j < TN; ++j) cmax[j] = -1.0e10f;
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m >= e.m_valid) continue;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (m >= e.m_valid) continue;
-            const float add = EPI == EPI_LOGL ? e.bias[m] : 0.0f;
-#pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                const int n = n0 + wn + j * 32 + (lane & 31);
-                const float v = acc[i][j][r] + add;
-                if (n < e.n_valid) {
-                    float *cp = e.C + (size_t)m * e.ldc + n;
-                    *cp = (EPI == EPI_STORE && e.accumulate) ? *cp + v : v;
+                for (int j = 0; j < TN; ++j) {
+                    const int n = n0 + wn + j * 32 + (lane & 31);
+                    if (n < e.n_valid) {
+                        float *cp = e.C + (size_t)m * e.ldc + n;
+                        *cp = e.accumulate ? *cp + acc[i][j][r] : acc[i][j][r];
+                    }
                 }
-                if (EPI == EPI_LOGL) cmax[j] = fmaxf(cmax[j], v);
             }
-        }
-    if (EPI != EPI_LOGL || !e.pm) return;
+    }
+}
 
-    // Softmax partials of this 256-row band (restates k_sm_partial's online max / sum of exp
-    // with the same -1e10 floors): column maxima over the lane pair, then over the 4 row-waves
-    // through LDS; then sums of exp(v - max) the same way.  The main loop's LDS is free.
-    float *red = reinterpret_cast<float *>(smem);            // [4][BN2] maxima
-    double *reds = reinterpret_cast<double *>(smem) + 2 * BN2;  // [4][BN2] sums (after the maxima)
-    const int wrow = wave >> 1;
+// ---- logL contraction with DMA operand staging -----------------------------------------------
+// Two-term A (log W hi, lo) x one-term BF16 B (counts), 256 x 256 macro tile, 8 wavefronts as in
+// k_gemm_mfma2, BK2 = 32.  The operands go from global memory straight into LDS through the
+// vector-memory DMA path (global_load_lds_dwordx4: lane l of a wave instruction lands at
+// M0 + 16 l), so no register staging and no LDS store instructions.  LDS rows are 64 B (32 BF16)
+// without padding; the four 16-byte chunks of row r sit at chunk ^ lds_swz(r), which spreads the
+// fragment reads of 8 (or 16) consecutive rows over distinct bank groups.  Three stages: the loads
+// of step k + 2 are issued during step k.  Each step reads the fragments of its second k-half
+// while the first half multiplies, and those of the next step's first half (after the barrier)
+// while the second half multiplies.
+// Measured at 25,856 x 20,224 x 3,200 against the register-staged k_gemm_mfma2: 5.70 vs 5.90 ms
+// and 5.86 vs 6.28 ms on two boxes, bit-identical logits and band partials.
+constexpr int kDmaStages = 3;
+
+__device__ __forceinline__ int lds_swz(int r) { return ((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 1); }
+
+// One DMA load of 16 bytes per lane into LDS at the wave-uniform byte address lds.  Written as
+// inline asm: with the builtin, the compiler's LDS-DMA tracking puts a vmcnt(0) in front of every
+// LDS read, which would wait for the loads of later stages too; the kernel waits with explicit
+// vmcnt counts instead.  The instruction has no register result.
+// M0 is a reserved register to the compiler; k_logl_dma's code uses it nowhere else (clang warns
+// about the clobber, which is silenced here).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void lds_dma16(const void *g, uint32_t lds)
+{
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off" ::"v"(g), "s"(lds) : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+__global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                                  int64_t lda, const bf16_t *__restrict__ B16, int64_t ldb, int Kd,
+                                                  int ntn, int gm, EpiArgs e)
+{
+    constexpr int NA = 2, BN2 = 256, TN = BN2 / 64;
+    constexpr int OPB = 256 * BK2 * 2;        // bytes of one operand per stage
+    constexpr int STAGEB = (NA + 1) * OPB;    // 48 KB
+    constexpr int LPW = (NA + 1) * 16 / 8;    // DMA instructions per wave per step (16 rows each)
+    constexpr uint32_t kVm0 = 0x0F70;         // s_waitcnt encoding: vmcnt(0), lgkmcnt and expcnt not waited
+    __shared__ __attribute__((aligned(1024))) unsigned char smem[kDmaStages * STAGEB];
+
+    // tile order as in k_gemm_mfma2
+    const int nwg = gridDim.x, b = blockIdx.x;
+    const int base = nwg / 8, rem = nwg % 8, xcd = b % 8, loc = b / 8;
+    const int t = xcd * base + min(xcd, rem) + loc;
+    const int ntm = nwg / ntn, gi = t / (gm * ntn), gr = t % (gm * ntn);
+    const int rows = min(gm, ntm - gi * gm);
+    const int m0 = (gi * gm + gr % rows) * BM2, n0 = (gr / rows) * BN2;
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN2 / 2);
+    // DMA instruction q of this wave: operand op (A hi, A lo, B), rows 16 rb .. 16 rb + 15; lane l
+    // fetches logical chunk (l & 3) ^ lds_swz(r) of row r = 16 rb + l / 4
+    const bf16_t *src[LPW];
+    int dst[LPW];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-        cmax[j] = fmaxf(cmax[j], __shfl_xor(cmax[j], 32, 64));
-        if (lane < 32) red[wrow * BN2 + wn + j * 32 + lane] = cmax[j];
+    for (int q = 0; q < LPW; ++q) {
+        const int g = wave * LPW + q, op = g >> 4, rb = g & 15;
+        const int r = rb * 16 + (lane >> 2), c = (lane & 3) ^ lds_swz(r);
+        const bf16_t *row = op == 0 ? A0 + (size_t)(m0 + r) * lda
+                          : op == 1 ? A1 + (size_t)(m0 + r) * lda
+                                    : B16 + (size_t)(n0 + r) * ldb;
+        src[q] = row + c * 8;
+        dst[q] = op * OPB + rb * 1024;
     }
-    __syncthreads();
+    const uint32_t lds0 = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) unsigned char *)smem;
+    auto issue = [&](int step, int stage) {
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-        const int cl = wn + j * 32 + (lane & 31);
-        cmax[j] = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
-    }
-    double csum[TN];
-#pragma unroll
-    for (int j = 0; j < TN; ++j) csum[j] = 0.0;
+        for (int q = 0; q < LPW; ++q) lds_dma16(src[q] + step * BK2, lds0 + stage * STAGEB + dst[q]);
+    };
+
+    f32x16 acc[2][TN];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (m >= e.m_valid) continue;
-            const float add = e.bias[m];
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                const float d = (acc[i][j][r] + add) - cmax[j];
-                if (d > -1.0e10f) csum[j] += (double)expf(d);
-            }
-        }
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const int fr = lane & 31, sw = lds_swz(fr & 15), c0 = lane >> 5;
+    struct Frag {
+        bf16x8 a[NA][2], b[TN];
+    };
+    auto rd = [&](Frag &F, int stage, int kk) {  // fragments of k-half kk (0 or 16) of a stage
+        const unsigned char *S = smem + stage * STAGEB;
+        const int co = ((c0 + kk / 8) ^ sw) * 16;
 #pragma unroll
-    for (int j = 0; j < TN; ++j) {
-        csum[j] += __shfl_xor(csum[j], 32, 64);
-        if (lane < 32) reds[wrow * BN2 + wn + j * 32 + lane] = csum[j];
-    }
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+                F.a[a][i] = *reinterpret_cast<const bf16x8 *>(S + a * OPB + (wm + i * 32 + fr) * 64 + co);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) F.b[j] = *reinterpret_cast<const bf16x8 *>(S + NA * OPB + (wn + j * 32 + fr) * 64 + co);
+    };
+    auto mm = [&](const Frag &F) {  // lo term first, as in k_gemm_mfma2
+#pragma unroll
+        for (int a = NA - 1; a >= 0; --a)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(F.a[a][i], F.b[j], acc[i][j], 0, 0, 0);
+    };
+
+    // stage of step k: k % 3.  The loads of step k + 2 go to the stage of step k - 1, whose reads
+    // all completed before the barrier in step k - 1.  The barrier in step k follows the wait for
+    // this wave's loads of step k + 1 (issued in step k - 1): LPW newer instructions may remain.
+    const int nk = Kd / BK2;
+    Frag F0, F1;
+    issue(0, 0);
+    if (nk > 1) issue(1, 1);
+    if (nk > 1) __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
+    else __builtin_amdgcn_s_waitcnt(kVm0);
     __syncthreads();
-    for (int cl = tid; cl < BN2; cl += 512) {
-        const int n = n0 + cl;
-        if (n >= e.n_valid) continue;
-        const float mx = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
-        const double sm = reds[cl] + reds[BN2 + cl] + reds[2 * BN2 + cl] + reds[3 * BN2 + cl];
-        const int64_t o = (e.band0 + m0 / BM2) * e.ldp + n;
-        e.pm[o] = mx;
-        e.ps[o] = sm;
+    rd(F0, 0, 0);
+    int k = 0;
+    for (; k + 2 < nk; ++k) {
+        rd(F1, k % kDmaStages, 16);
+        issue(k + 2, (k + 2) % kDmaStages);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(F0);
+        __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
+        __syncthreads();
+        rd(F0, (k + 1) % kDmaStages, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(F1);
     }
+    if (k + 1 < nk) {
+        rd(F1, k % kDmaStages, 16);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(F0);
+        __builtin_amdgcn_s_waitcnt(kVm0);
+        __syncthreads();
+        rd(F0, (k + 1) % kDmaStages, 0);
+        __builtin_amdgcn_sched_barrier(0);
+        mm(F1);
+        ++k;
+    }
+    rd(F1, k % kDmaStages, 16);
+    mm(F0);
+    mm(F1);
+    __syncthreads();  // the epilogue reuses the LDS
+    logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem);
 }
 
 template <int NA, int EPI, int BN2, bool BU8>
@@ -503,6 +672,25 @@ hipError_t run_large(int M, int N, hipStream_t st, int na, const bf16_t *A0, con
 
 }  // namespace
 
+// logL with the DMA kernel over the 256-wide column tiles (K16 operand), plus the 128-wide
+// register-staged remainder launch when N % 256 = 128.
+hipError_t run_logl_dma(int M, int N, hipStream_t st, const bf16_t *A0, const bf16_t *A1, int64_t lda,
+                        const bf16_t *B16, const uint8_t *B8, int64_t ldb, int Kd, const EpiArgs &e)
+{
+    const int ntm = M / BM2, n256 = N / 256;
+    hipError_t err = hipSuccess;
+    if (n256 > 0) {
+        const int gm = std::max(1, std::min(e.tile_rows > 0 ? e.tile_rows : kTileRows, ntm));
+        hipLaunchKernelGGL(k_logl_dma, dim3(ntm * n256), dim3(512), 0, st, A0, A1, lda, B16, ldb, Kd, n256, gm, e);
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess && N % 256) {
+        if (B8) err = run2<2, EPI_LOGL, 128, true>(ntm, 1, n256 * 256, st, A0, A1, nullptr, lda, B8, ldb, Kd, e);
+        else err = run2<2, EPI_LOGL, 128, false>(ntm, 1, n256 * 256, st, A0, A1, nullptr, lda, B16, ldb, Kd, e);
+    }
+    return err;
+}
+
 bool logl_fuses_softmax(const emc_ctx *c) { return c->mfma && !c->chi2_mfma && !c->k_split && c->opt.mfma_tile == 0; }
 
 int softmax_fused_bands(const emc_ctx *c) { return (int)((int64_t)((c->S + c->chunk - 1) / c->chunk) * (c->chunkp / BM2)); }
@@ -526,6 +714,8 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
         e.ps = c->part + nb * c->I;
         e.band0 = (int64_t)(s0 / c->chunk) * (c->chunkp / BM2);
         e.ldp = c->I;
+        if (c->la_terms == 2 && c->opt.logl_staging == 0)
+            return run_logl_dma(c->chunkp, c->Ip, st, c->Lh, A1, c->Pp, c->K16, c->K8, c->Pp, c->Pp, e);
         return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->K8, c->Pp,
                                    c->Pp, e);
     }

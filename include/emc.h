@@ -100,7 +100,11 @@ typedef struct emc_options {
                                 precision).  0 = phi = 1 (known_intensity = true) */
     int32_t tile_rows;       /* large-tile MFMA contractions: row panels per group of consecutive tiles
                                 (L2 reuse within an XCD); 0 = the built-in default */
-    int32_t reserved[2];
+    int32_t logl_staging;    /* 256-row logL contraction with a two-term log W and a one-term counts
+                                operand: 0 = operands staged in LDS by direct global->LDS loads (the
+                                BF16 counts), 1 = staged through registers (byte counts widened in
+                                registers).  Same sums in the same order: identical logits */
+    int32_t reserved[1];
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone

@@ -405,6 +405,25 @@ def test_byte_counts_equal_bf16_counts(small):
     assert np.allclose(outs[0][1], outs[1][1], rtol=1e-5, atol=1e-7)  # insertion order of float atomics varies
 
 
+@pytest.mark.parametrize("chunk", [0, 300])
+def test_logl_dma_staging_equals_register_staging(small, chunk):
+    """The logL contraction with operands staged in LDS by direct global->LDS loads gives the same
+    logits, log-sum-exps and responsibilities, bit for bit, as the register-staged kernel (same
+    products, same order), unchunked and over rotation chunks (band offsets of the softmax partials)."""
+    log(f"[logl staging] chunk={chunk}")
+    d = dict(small)
+    d["K"] = d["K"][:700]
+    outs = []
+    for st in (0, 1):
+        with engine(d, "bf16x2", S=1300, I=700, slice_chunk=chunk, logl_staging=st) as e:
+            e.logl()
+            R = e.get_logl()
+            e.normalize_resp()
+            outs.append((R, e.get_resp(), e.get_lse()))
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
 def test_sparse_mstep_with_dense_rows_equals_dense(small):
     """Rows with thousands of non-zero responsibilities (9,000 patterns, 70 % non-zero): the sparse
     form's LDS list fills and is flushed several times within one scan step; it still gives the
