@@ -384,6 +384,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
+    ap.add_argument("--logl-staging", type=int, default=0, choices=[0, 1],
+                    help="logL contraction: 0 = DMA-staged operands (default), 1 = register-staged")
     ap.add_argument("--mstep", default="auto", choices=["auto", "dense", "sparse"],
                     help="M-step form of the headline run (auto: chosen on the device per iteration)")
     ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
@@ -422,7 +424,8 @@ def main():
     log(f"[bench] workload {args.workload}: S={S} I={I} L={L} P_act={Pa} max count {d['K'].max()} "
         f"(generated in {time.perf_counter() - t:.1f}s)")
 
-    ms_mfma, st_mfma, last = run_engine(d, "bf16x2", args.steps, args.warmup, args.insert, mstep=args.mstep)
+    ms_mfma, st_mfma, last = run_engine(d, "bf16x2", args.steps, args.warmup, args.insert, mstep=args.mstep,
+                                        logl_staging=args.logl_staging)
     fp32 = None
     if not args.no_fp32:
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
@@ -449,10 +452,13 @@ def main():
     flops = 2.0 * I * S * Pa  # one logical K.logW^T product, counted once (the BF16 split is overhead)
     t_logl = st_mfma["logl_gemm"] * 1e-3
     achieved = flops / t_logl / 1e12
-    roof = {"kernel": "k_gemm_mfma2 (logL contraction, 256x256 tiles, byte counts)", "bound": "mfma",
+    kname, kprof = (("k_logl_dma (logL contraction, 256x256 tiles, DMA-staged BF16 operands)", "logl_dma")
+                    if args.logl_staging == 0 else
+                    ("k_gemm_mfma2 (logL contraction, 256x256 tiles, register-staged byte counts)", "logl_mfma"))
+    roof = {"kernel": kname, "bound": "mfma",
             "achieved": achieved, "peak": PEAK["bf16_tflops"], "unit": "TFLOP/s",
             "frac": achieved / PEAK["bf16_tflops"],
-            "traffic": traffic_from_profile("logl_mfma", args.workload),
+            "traffic": traffic_from_profile(kprof, args.workload),
             "algorithmic": f"2*I*S*P_act = {flops:.4e} FLOP per launch (one launch per iteration)",
             "launch_ms": st_mfma["logl_gemm"]}
 
