@@ -176,6 +176,38 @@ def test_trained_network_captures_mass_and_beats_untrained(d):
             assert rel < 5e-2
 
 
+def test_network_adds_capture_to_recall_at_lower_cost(d):
+    """Ablation of the network inside the policy: the trained network's top-k plus the recall
+    against the same policy with the network frozen at its random initialisation (Adam step
+    1e-12, one pass), both from the random initial model, no fallback, so every proposal
+    iteration searches exactly the union of the top-k and recalled cells.  The frozen arm is the
+    recall plus k cells that carry no information about the pattern.  The trained arm captures
+    more of the full search's responsibility mass while the model forms and searches fewer
+    (rotation, pattern) pairs: its top-k overlaps the recalled cells."""
+    k, R, n_it = 2, 2, 14
+    log(f"[ablation] top_k={k} + recall={R} of 60 cells, trained vs frozen network, random start")
+    m0 = wl.normalized(d["model0"]).astype(np.float32)
+    arms = {}
+    for arm, extra in (("trained", {}), ("frozen", dict(epochs=1, refresh_epochs=0, lr=1e-12))):
+        with engine(d, model=m0, blur_sigma=0.5) as e:
+            e.enable_proposal(d["cq"], top_k=k, recall=R, warmup_full=2, full_every=3, min_mass=0.0,
+                              max_fallback=1.0, hidden=256, batch=128, seed=5, **extra)
+            caps, work = [], []
+            for it in range(n_it):
+                s = e.iterate(it)
+                if s["search"] == "full" and s["captured_mass"] >= 0:
+                    caps.append(s["captured_mass"])
+                elif s["search"] == "proposal":
+                    work.append(s["work_fraction"])
+        arms[arm] = (np.array(caps[:4]), float(np.mean(work)))
+        log(f"  {arm}: captured at the full searches {np.round(caps, 4).tolist()}, "
+            f"mean work fraction of the proposal iterations {np.mean(work):.4f}")
+    (ct, wt), (cf, wf) = arms["trained"], arms["frozen"]
+    assert ct.size == 4 and cf.size == 4
+    assert ct.sum() > cf.sum() + 0.05
+    assert wt < wf
+
+
 def check_top_logit_cells(e, d, rec, R):
     """rec [I][R] holds, for each pattern, R distinct cells of largest max logit (the logits of
     the last full search; a tolerance of a few float roundings at the R-th value)."""
