@@ -132,6 +132,20 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         lh[k] = (float)l;
         ll[k] = (float)(l - (double)lh[k]);
     }
+    // every row's logits and segment flag are loaded before the first is used, so the wave waits for
+    // memory once rather than once per row
+    // (rows past S read row S - 1 and a segment past the last reads the last; their values are not
+    // used)
+    float4 vin[kApplyRows];
+    int flag[kApplyRows];
+    const int segc = OUT == 1 ? min(seg, nseg - 1) : 0;
+#pragma unroll
+    for (int rr = 0; rr < kApplyRows; ++rr) {
+        const size_t sc = (size_t)min(r0 + rr, S - 1);
+        vin[rr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (c0 < I) vin[rr] = *reinterpret_cast<const float4 *>(R + sc * ldr + c0);
+        flag[rr] = OUT == 1 ? (int)pdirty[sc * nseg + segc] : 0;
+    }
     double rowsum[kApplyRows];
 #pragma unroll
     for (int rr = 0; rr < kApplyRows; ++rr) {
@@ -141,7 +155,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         int any_nz = 0;
         if (s < S && c0 < I) {
             float *src = R + (size_t)s * ldr + c0;
-            const float4 v = *reinterpret_cast<const float4 *>(src);
+            const float4 v = vin[rr];
             const float x[4] = {v.x, v.y, v.z, v.w};
             // exp(d) = 2^(d log2 e) on the hardware exp2: relative error about |d| 6e-8, below the
             // 2^-16 of the BF16 pair P is stored as
@@ -177,7 +191,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         if (OUT == 1 && s < S && seg_live) {  // s and seg_live are uniform over the wave
             const bool seg_nz = __any(any_nz) != 0;
             const size_t fo = (size_t)s * nseg + seg;
-            const bool dirty = pdirty[fo] != 0;
+            const bool dirty = flag[rr] != 0;
             if ((seg_nz || dirty) && c0 < I) {
                 const size_t o = (size_t)s * ldr + c0;
                 *reinterpret_cast<uint2 *>(Ph + o) = hv;
