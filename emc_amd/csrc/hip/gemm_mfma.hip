@@ -266,10 +266,12 @@ __device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
 // then (when e.pm is set) the softmax partials of this 256-row band, restating k_sm_partial's
 // online max / sum of exp with the same -1e10 floors: column maxima over the lane pair, then over
 // the 4 row-waves through LDS; then sums of exp(v - max) the same way.  lds: >= 12 BN2 bytes,
-// free (the main loop's stages).
-template <int TN, int BN2>
+// free (the main loop's stages).  kBiasLds: the tile's 256 row biases are read from bias_lds (the
+// kernel's LDS copy) instead of global memory, so no vector-memory wait holds up the logit stores
+// (on gfx9 a wait for a load also waits for every store issued before it).
+template <int TN, int BN2, bool kBiasLds = false>
 __device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm,
-                                              int wn, void *lds)
+                                              int wn, void *lds, const float *bias_lds = nullptr)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float cmax[TN];
@@ -281,7 +283,9 @@ __device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const 
         for (int r = 0; r < 16; ++r) {
             const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             if (m >= e.m_valid) continue;
-            const float add = e.bias[m];
+            float add;
+            if constexpr (kBiasLds) add = bias_lds[m - m0];
+            else add = e.bias[m];
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const int n = n0 + wn + j * 32 + (lane & 31);
@@ -314,7 +318,9 @@ __device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const 
         for (int r = 0; r < 16; ++r) {
             const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
             if (m >= e.m_valid) continue;
-            const float add = e.bias[m];
+            float add;
+            if constexpr (kBiasLds) add = bias_lds[m - m0];
+            else add = e.bias[m];
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
                 const float d = (acc[i][j][r] + add) - cmax[j];
@@ -525,6 +531,7 @@ __global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0,
     constexpr int LPW = (NA + 1) * 16 / 8;    // DMA instructions per wave per step (16 rows each)
     constexpr uint32_t kVm0 = 0x0F70;         // s_waitcnt encoding: vmcnt(0), lgkmcnt and expcnt not waited
     __shared__ __attribute__((aligned(1024))) unsigned char smem[kDmaStages * STAGEB];
+    __shared__ float bias_s[BM2];
 
     // tile order as in k_gemm_mfma2
     const int nwg = gridDim.x, b = blockIdx.x;
@@ -594,10 +601,12 @@ __global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0,
     // this wave's loads of step k + 1 (issued in step k - 1): LPW newer instructions may remain.
     const int nk = Kd / BK2;
     Frag F0, F1;
+    const float bv = tid < BM2 && m0 + tid < e.m_valid ? e.bias[m0 + tid] : 0.0f;
     issue(0, 0);
     if (nk > 1) issue(1, 1);
     if (nk > 1) __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
     else __builtin_amdgcn_s_waitcnt(kVm0);
+    if (tid < BM2) bias_s[tid] = bv;
     __syncthreads();
     rd(F0, 0, 0);
     int k = 0;
@@ -627,7 +636,7 @@ __global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0,
     mm(F0);
     mm(F1);
     __syncthreads();  // the epilogue reuses the LDS
-    logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem);
+    logl_epilogue<TN, BN2, true>(acc, e, m0, n0, wm, wn, smem, bias_s);
 }
 
 template <int NA, int EPI, int BN2, bool BU8>
