@@ -384,7 +384,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
-    ap.add_argument("--logl-staging", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--mfma-staging", type=int, default=0, choices=[0, 1],
                     help="logL contraction: 0 = DMA-staged operands (default), 1 = register-staged")
     ap.add_argument("--mstep", default="auto", choices=["auto", "dense", "sparse"],
                     help="M-step form of the headline run (auto: chosen on the device per iteration)")
@@ -425,7 +425,7 @@ def main():
         f"(generated in {time.perf_counter() - t:.1f}s)")
 
     ms_mfma, st_mfma, last = run_engine(d, "bf16x2", args.steps, args.warmup, args.insert, mstep=args.mstep,
-                                        logl_staging=args.logl_staging)
+                                        mfma_staging=args.mfma_staging)
     fp32 = None
     if not args.no_fp32:
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
@@ -453,7 +453,7 @@ def main():
     t_logl = st_mfma["logl_gemm"] * 1e-3
     achieved = flops / t_logl / 1e12
     kname, kprof = (("k_logl_dma (logL contraction, 256x256 tiles, DMA-staged BF16 operands)", "logl_dma")
-                    if args.logl_staging == 0 else
+                    if args.mfma_staging == 0 else
                     ("k_gemm_mfma2 (logL contraction, 256x256 tiles, register-staged byte counts)", "logl_mfma"))
     roof = {"kernel": kname, "bound": "mfma",
             "achieved": achieved, "peak": PEAK["bf16_tflops"], "unit": "TFLOP/s",

@@ -37,7 +37,7 @@ class Options(C.Structure):
                 ("device", C.c_int32), ("blur_sigma", C.c_double), ("fluence", C.c_double),
                 ("timing", C.c_int32), ("mfma_tile", C.c_int32), ("count_bytes", C.c_int32),
                 ("mstep", C.c_int32), ("scaling", C.c_int32), ("tile_rows", C.c_int32),
-                ("logl_staging", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("mfma_staging", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class IterStats(C.Structure):

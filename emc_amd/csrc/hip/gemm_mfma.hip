@@ -521,10 +521,13 @@ __device__ __forceinline__ void lds_dma16(const void *g, uint32_t lds)
 }
 #pragma clang diagnostic pop
 
-__global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
-                                                  int64_t lda, const bf16_t *__restrict__ B16, int64_t ldb, int Kd,
-                                                  int ntn, int gm, EpiArgs e)
+template <int EPI>
+__device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                              int64_t lda, const bf16_t *__restrict__ B16, int64_t ldb, int Kd,
+                                              int ntn, int gm, const EpiArgs &e)
 {
+    static_assert(EPI == EPI_LOGL || EPI == EPI_STORE, "logL or store epilogue");
+    if (e.gate && *e.gate <= e.gate_thr) return;  // the sparse M-step runs instead
     constexpr int NA = 2, BN2 = 256, TN = BN2 / 64;
     constexpr int OPB = 256 * BK2 * 2;        // bytes of one operand per stage
     constexpr int STAGEB = (NA + 1) * OPB;    // 48 KB
@@ -601,7 +604,7 @@ __global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0,
     // this wave's loads of step k + 1 (issued in step k - 1): LPW newer instructions may remain.
     const int nk = Kd / BK2;
     Frag F0, F1;
-    const float bv = tid < BM2 && m0 + tid < e.m_valid ? e.bias[m0 + tid] : 0.0f;
+    const float bv = EPI == EPI_LOGL && tid < BM2 && m0 + tid < e.m_valid ? e.bias[m0 + tid] : 0.0f;
     issue(0, 0);
     if (nk > 1) issue(1, 1);
     if (nk > 1) __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
@@ -636,7 +639,41 @@ __global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0,
     mm(F0);
     mm(F1);
     __syncthreads();  // the epilogue reuses the LDS
-    logl_epilogue<TN, BN2, true>(acc, e, m0, n0, wm, wn, smem, bias_s);
+    if constexpr (EPI == EPI_LOGL) {
+        logl_epilogue<TN, BN2, true>(acc, e, m0, n0, wm, wn, smem, bias_s);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m >= e.m_valid) continue;
+#pragma unroll
+                for (int j = 0; j < TN; ++j) {
+                    const int n = n0 + wn + j * 32 + (lane & 31);
+                    if (n < e.n_valid) {
+                        float *cp = e.C + (size_t)m * e.ldc + n;
+                        *cp = e.accumulate ? *cp + acc[i][j][r] : acc[i][j][r];
+                    }
+                }
+            }
+    }
+}
+
+// logL: R = log W . K^T + bias with the softmax band partials; store: C (=|+=) A . B^T (the dense
+// M-step U = P^T . K and the chi^2 contractions).
+__global__ __launch_bounds__(512) void k_logl_dma(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                                  int64_t lda, const bf16_t *__restrict__ B16, int64_t ldb, int Kd,
+                                                  int ntn, int gm, EpiArgs e)
+{
+    mfma_dma_body<EPI_LOGL>(A0, A1, lda, B16, ldb, Kd, ntn, gm, e);
+}
+
+__global__ __launch_bounds__(512) void k_store_dma(const bf16_t *__restrict__ A0, const bf16_t *__restrict__ A1,
+                                                   int64_t lda, const bf16_t *__restrict__ B16, int64_t ldb, int Kd,
+                                                   int ntn, int gm, EpiArgs e)
+{
+    mfma_dma_body<EPI_STORE>(A0, A1, lda, B16, ldb, Kd, ntn, gm, e);
 }
 
 template <int NA, int EPI, int BN2, bool BU8>
@@ -681,21 +718,23 @@ hipError_t run_large(int M, int N, hipStream_t st, int na, const bf16_t *A0, con
 
 }  // namespace
 
-// logL with the DMA kernel over the 256-wide column tiles (K16 operand), plus the 128-wide
-// register-staged remainder launch when N % 256 = 128.
-hipError_t run_logl_dma(int M, int N, hipStream_t st, const bf16_t *A0, const bf16_t *A1, int64_t lda,
-                        const bf16_t *B16, const uint8_t *B8, int64_t ldb, int Kd, const EpiArgs &e)
+// Two-term A x one-term B with the DMA kernels over the 256-wide column tiles (BF16 B operand),
+// plus the 128-wide register-staged remainder launch when N % 256 = 128.
+template <int EPI>
+hipError_t run_dma(int M, int N, hipStream_t st, const bf16_t *A0, const bf16_t *A1, int64_t lda,
+                   const bf16_t *B16, const uint8_t *B8, int64_t ldb, int Kd, const EpiArgs &e)
 {
     const int ntm = M / BM2, n256 = N / 256;
     hipError_t err = hipSuccess;
     if (n256 > 0) {
         const int gm = std::max(1, std::min(e.tile_rows > 0 ? e.tile_rows : kTileRows, ntm));
-        hipLaunchKernelGGL(k_logl_dma, dim3(ntm * n256), dim3(512), 0, st, A0, A1, lda, B16, ldb, Kd, n256, gm, e);
+        hipLaunchKernelGGL(EPI == EPI_LOGL ? k_logl_dma : k_store_dma, dim3(ntm * n256), dim3(512), 0, st, A0, A1, lda,
+                           B16, ldb, Kd, n256, gm, e);
         err = hipGetLastError();
     }
     if (err == hipSuccess && N % 256) {
-        if (B8) err = run2<2, EPI_LOGL, 128, true>(ntm, 1, n256 * 256, st, A0, A1, nullptr, lda, B8, ldb, Kd, e);
-        else err = run2<2, EPI_LOGL, 128, false>(ntm, 1, n256 * 256, st, A0, A1, nullptr, lda, B16, ldb, Kd, e);
+        if (B8) err = run2<2, EPI, 128, true>(ntm, 1, n256 * 256, st, A0, A1, nullptr, lda, B8, ldb, Kd, e);
+        else err = run2<2, EPI, 128, false>(ntm, 1, n256 * 256, st, A0, A1, nullptr, lda, B16, ldb, Kd, e);
     }
     return err;
 }
@@ -723,8 +762,8 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
         e.ps = c->part + nb * c->I;
         e.band0 = (int64_t)(s0 / c->chunk) * (c->chunkp / BM2);
         e.ldp = c->I;
-        if (c->la_terms == 2 && c->opt.logl_staging == 0)
-            return run_logl_dma(c->chunkp, c->Ip, st, c->Lh, A1, c->Pp, c->K16, c->K8, c->Pp, c->Pp, e);
+        if (c->la_terms == 2 && c->opt.mfma_staging == 0)
+            return run_dma<EPI_LOGL>(c->chunkp, c->Ip, st, c->Lh, A1, c->Pp, c->K16, c->K8, c->Pp, c->Pp, e);
         return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->K8, c->Pp,
                                    c->Pp, e);
     }
@@ -747,6 +786,8 @@ hipError_t launch_gemm_store_mfma(const emc_ctx *c, int na, const bf16_t *A0, co
     e.m_valid = c->chunkp;
     e.n_valid = c->Ip;
     e.accumulate = accumulate;
+    if (na == 2 && B16 && c->opt.mfma_staging == 0)
+        return run_dma<EPI_STORE>(c->chunkp, c->Ip, st, A0, A1, c->Pp, B16, B8, c->Pp, c->Pp, e);
     return run_large<EPI_STORE>(c->chunkp, c->Ip, st, na, A0, A1, A2, c->Pp, B16, B8, c->Pp, c->Pp, e);
 }
 
@@ -777,6 +818,8 @@ hipError_t launch_mstep_mfma(const emc_ctx *c, int s0, int cs, bool fused, const
         return run<2, 1, EPI_SCATTER>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);
     }
     if (c->k_split) return run<2, 2, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, c->KT16lo, c->Ip, c->Ip, e);
+    if (c->opt.mfma_tile == 0 && c->opt.mfma_staging == 0)
+        return run_dma<EPI_STORE>(c->chunkp, c->Pp, st, Ah, Al, c->Ip, c->KT16, c->KT8, c->Ip, c->Ip, e);
     if (c->opt.mfma_tile == 0)
         return run_large<EPI_STORE>(c->chunkp, c->Pp, st, 2, Ah, Al, nullptr, c->Ip, c->KT16, c->KT8, c->Ip, c->Ip, e);
     return run<2, 1, EPI_STORE>(grid, st, Ah, Al, nullptr, c->Ip, c->KT16, nullptr, c->Ip, c->Ip, e);

@@ -100,10 +100,11 @@ typedef struct emc_options {
                                 precision).  0 = phi = 1 (known_intensity = true) */
     int32_t tile_rows;       /* large-tile MFMA contractions: row panels per group of consecutive tiles
                                 (L2 reuse within an XCD); 0 = the built-in default */
-    int32_t logl_staging;    /* 256-row logL contraction with a two-term log W and a one-term counts
-                                operand: 0 = operands staged in LDS by direct global->LDS loads (the
-                                BF16 counts), 1 = staged through registers (byte counts widened in
-                                registers).  Same sums in the same order: identical logits */
+    int32_t mfma_staging;    /* 256-row MFMA contractions of a two-term operand with a one-term one
+                                (logL, dense M-step, chi^2 sums): 0 = operands staged in LDS by direct
+                                global->LDS loads (BF16 counts), 1 = staged through registers (byte
+                                counts widened in registers).  Same products in the same order:
+                                identical results */
     int32_t reserved[1];
 } emc_options;
 

@@ -405,21 +405,27 @@ def test_byte_counts_equal_bf16_counts(small):
     assert np.allclose(outs[0][1], outs[1][1], rtol=1e-5, atol=1e-7)  # insertion order of float atomics varies
 
 
-@pytest.mark.parametrize("chunk", [0, 300])
-def test_logl_dma_staging_equals_register_staging(small, chunk):
-    """The logL contraction with operands staged in LDS by direct global->LDS loads gives the same
-    logits, log-sum-exps and responsibilities, bit for bit, as the register-staged kernel (same
-    products, same order), unchunked and over rotation chunks (band offsets of the softmax partials)."""
-    log(f"[logl staging] chunk={chunk}")
+@pytest.mark.parametrize("chunk,diff", [(0, "true_poisson"), (300, "true_poisson"), (0, "poisson")])
+def test_dma_staging_equals_register_staging(small, chunk, diff):
+    """The MFMA contractions with operands staged in LDS by direct global->LDS loads give the same
+    logits, log-sum-exps, responsibilities and (dense M-step) model, bit for bit, as the
+    register-staged kernels (same products, same order): unchunked and over rotation chunks (band
+    offsets of the softmax partials), and for the chi^2 kind with intensity scaling (its four
+    contractions through the store epilogue)."""
+    log(f"[mfma staging] chunk={chunk} diff={diff}")
     d = dict(small)
     d["K"] = d["K"][:700]
+    kw = dict(diff=diff, scaling=1) if diff != "true_poisson" else {}
     outs = []
     for st in (0, 1):
-        with engine(d, "bf16x2", S=1300, I=700, slice_chunk=chunk, logl_staging=st) as e:
+        with engine(d, "bf16x2", S=1300, I=700, slice_chunk=chunk, mfma_staging=st, mstep="dense", **kw) as e:
             e.logl()
             R = e.get_logl()
             e.normalize_resp()
-            outs.append((R, e.get_resp(), e.get_lse()))
+            P, lse = e.get_resp(), e.get_lse()
+            e.maximize()
+            e.compress()
+            outs.append((R, P, lse, e.get_model()[0]))
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
 
