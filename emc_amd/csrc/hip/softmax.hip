@@ -103,6 +103,16 @@ __global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, 
 }
 
 constexpr int kApplyRows = 8;    // rotation rows per workgroup
+
+// Two floats to a packed BF16 pair (a in the low half), round to nearest even: gfx950's
+// two-source v_cvt_pk_bf16_f32, which rounds as f2bf_dev does.  Written as asm because the compiler
+// emits one single-source conversion per value and merges the halves with extra instructions.
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b)
+{
+    uint32_t r;
+    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
 
 // OUT = 0: P in place (FP32).  OUT = 1: BF16 hi/lo pair, of P * iphi when iphi is given (the
@@ -125,12 +135,13 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
     const bool seg_live = seg * 256 < I;                                    // uniform over the wave
     // lse as a float pair: x - lh is exact wherever p is not negligible (x and lse within a factor
     // of 2, Sterbenz), so exp((x - lh) - ll) carries the double-precision argument to float rounding
+    // (a column past I gets lh = +inf: its p is exp2(-inf) = 0 without a select per element)
     float lh[4], ll[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const double l = (c0 + k < I) ? lse[c0 + k] : 0.0;
-        lh[k] = (float)l;
-        ll[k] = (float)(l - (double)lh[k]);
+        lh[k] = (c0 + k < I) ? (float)l : INFINITY;
+        ll[k] = (c0 + k < I) ? (float)(l - (double)lh[k]) : 0.0f;
     }
     // every row's logits and segment flag are loaded before the first is used, so the wave waits for
     // memory once rather than once per row
@@ -162,29 +173,29 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
             constexpr float kLog2e = 1.4426950408889634f;
             float p[4];
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                p[k] = (c0 + k < I) ? __builtin_amdgcn_exp2f(((x[k] - lh[k]) - ll[k]) * kLog2e) : 0.0f;
+            for (int k = 0; k < 4; ++k) p[k] = __builtin_amdgcn_exp2f(((x[k] - lh[k]) - ll[k]) * kLog2e);
             acc = (double)((p[0] + p[1]) + (p[2] + p[3]));
             if (OUT == 0) {
                 *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
             } else {
-                bf16_t h[4], lo[4];
                 const size_t o = (size_t)s * ldr + c0;
                 float q[4] = {p[0], p[1], p[2], p[3]};
                 if (iphi) {
                     const float4 f = *reinterpret_cast<const float4 *>(iphi + o);
                     q[0] *= f.x; q[1] *= f.y; q[2] *= f.z; q[3] *= f.w;
                 }
-                int mine = 0;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    h[k] = f2bf_dev(q[k]);
-                    lo[k] = f2bf_dev(q[k] - bf2f(h[k]));
-                    mine += (h[k] | lo[k]) != 0;
-                }
+                // hi of a pair in one v_cvt_pk_bf16_f32; its two values back as floats by a shift and a
+                // mask; lo = bf16(q - hi).  q >= 0, so (hi | lo) != 0 exactly when hi != 0.
+                const uint32_t h01 = pk_bf16(q[0], q[1]), h23 = pk_bf16(q[2], q[3]);
+                const uint32_t l01 = pk_bf16(q[0] - __builtin_bit_cast(float, h01 << 16),
+                                             q[1] - __builtin_bit_cast(float, h01 & 0xffff0000u));
+                const uint32_t l23 = pk_bf16(q[2] - __builtin_bit_cast(float, h23 << 16),
+                                             q[3] - __builtin_bit_cast(float, h23 & 0xffff0000u));
+                const int mine = ((h01 & 0xffffu) != 0) + ((h01 >> 16) != 0) + ((h23 & 0xffffu) != 0) +
+                                 ((h23 >> 16) != 0);
                 nz += mine;
-                hv = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
-                lv = make_uint2((uint32_t)lo[0] | ((uint32_t)lo[1] << 16), (uint32_t)lo[2] | ((uint32_t)lo[3] << 16));
+                hv = make_uint2(h01, h23);
+                lv = make_uint2(l01, l23);
                 any_nz = mine != 0;
             }
         }
