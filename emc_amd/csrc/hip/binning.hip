@@ -38,14 +38,14 @@ namespace {
 
 constexpr int kBrick = 8;
 constexpr int kBrick3 = kBrick * kBrick * kBrick;  // base voxels of a brick
-constexpr int kUnit = 2048;                        // pairs per workgroup (positions fit 16 bits)
+constexpr int kUnit = 4096;                        // pairs per workgroup (positions fit 16 bits)
 constexpr int kVbeg = 520;                         // run starts per unit: kBrick3 + 1, padded to 16 bytes
 constexpr int kUnitsPerBlock = 4;                  // units one insertion workgroup processes in turn
 constexpr int kGroup = 32;                         // blocks whose partials one grouping workgroup sums
 constexpr int kHalo = kBrick + 1;
 constexpr int kHalo3 = kHalo * kHalo * kHalo;      // voxels a brick's pairs reach
 constexpr int kStage = 8;                          // pairs per thread whose loads are issued together
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;  // 8 waves; with 67 KB of LDS two workgroups (16 waves) fit a CU
 static_assert(kUnit % kThreads == 0, "step 2 gives every thread the same share");
 
 struct BinGeom {
