@@ -14,7 +14,9 @@
 namespace emc {
 namespace {
 
-constexpr int kStatsG = 16;  // rotations per workgroup in the statistics pass
+// rotations per workgroup in the statistics pass: 8 keeps 4 waves per SIMD (16 needed 210 VGPRs,
+// 2 waves per SIMD: 0.77 instead of 0.61 ms at 64^3 / 25,680 rotations)
+constexpr int kStatsG = 8;
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
