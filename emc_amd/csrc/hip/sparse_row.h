@@ -55,6 +55,9 @@ __device__ __forceinline__ void sparse_row(const bf16_t *__restrict__ ph, const 
         for (int q = 0; q < NPASS; ++q) {
             const int pc = tid + q * 256;
             if (pc >= nch) break;
+            // unrolled so that the count-row loads of several entries are in flight together (the
+            // sums still add the entries in list order)
+#pragma unroll 4
             for (int e = 0; e < cnt; ++e) {
                 const size_t row = (size_t)li[e] * Pp + 16 * (size_t)pc;
                 const float p = lp[e];
