@@ -229,44 +229,55 @@ __global__ __launch_bounds__(256) void k_slice_bf16(const float *__restrict__ mo
 
 // The same operand split from the log W kept by the statistics pass: x = (log W + log phi0, or 0
 // where W <= 0) - colmean[j], the same float operations as k_slice_bf16, so the terms are
-// bit-identical.  Four pixels per thread (Pp is a multiple of 128).
+// bit-identical.  Four pixels per thread and kSplitRows rows per workgroup, whose loads are issued
+// together (the column means are read once for all of them).  Pp is a multiple of 128.
+constexpr int kSplitRows = 8;
+
 template <int NT>
 __global__ __launch_bounds__(256) void k_split_bf16(const float *__restrict__ lw, int s0, int cs, int Pa, int Pp,
                                                     const float *__restrict__ colmean,
                                                     const double *__restrict__ scal, bf16_t *__restrict__ Lh,
                                                     bf16_t *__restrict__ Lm, bf16_t *__restrict__ Ll)
 {
-    const int r = blockIdx.y;
+    const int r0 = blockIdx.y * kSplitRows;
     const int j0 = (blockIdx.x * 256 + threadIdx.x) * 4;
     if (j0 >= Pp) return;
-    const bool row_ok = r < cs;
     const float logphi0 = (float)scal[SC_LOGPHI0];
-    float l[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
-    if (row_ok) {
-        const float4 v = *reinterpret_cast<const float4 *>(lw + (size_t)(s0 + r) * Pp + j0);
-        l[0] = v.x; l[1] = v.y; l[2] = v.z; l[3] = v.w;
-    }
-    uint16_t h[4], mi[4], lo[4];
+    float cm[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int j = j0 + q;
-        float x = 0.0f;
-        if (row_ok && j < Pa) x = (l[q] > -INFINITY ? l[q] + logphi0 : 0.0f) - colmean[j];
-        h[q] = f2bf(x);
-        const float r1 = x - bf2f(h[q]);
-        mi[q] = f2bf(r1);
-        lo[q] = f2bf(r1 - bf2f(mi[q]));
+    for (int q = 0; q < 4; ++q) cm[q] = j0 + q < Pa ? colmean[j0 + q] : 0.0f;
+    float4 v[kSplitRows];
+#pragma unroll
+    for (int rr = 0; rr < kSplitRows; ++rr) {
+        v[rr] = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+        if (r0 + rr < cs) v[rr] = *reinterpret_cast<const float4 *>(lw + (size_t)(s0 + r0 + rr) * Pp + j0);
     }
-    auto pack = [](const uint16_t *v) {
-        return make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+    auto pack = [](const uint16_t *w) {
+        return make_uint2((uint32_t)w[0] | ((uint32_t)w[1] << 16), (uint32_t)w[2] | ((uint32_t)w[3] << 16));
     };
-    const size_t o = (size_t)r * Pp + j0;
-    *reinterpret_cast<uint2 *>(Lh + o) = pack(h);
-    if (NT == 2) {
-        *reinterpret_cast<uint2 *>(Ll + o) = pack(mi);
-    } else {
-        *reinterpret_cast<uint2 *>(Lm + o) = pack(mi);
-        *reinterpret_cast<uint2 *>(Ll + o) = pack(lo);
+#pragma unroll
+    for (int rr = 0; rr < kSplitRows; ++rr) {
+        const int r = r0 + rr;  // rows of the padded chunk past cs are written as zeros
+        const float l[4] = {v[rr].x, v[rr].y, v[rr].z, v[rr].w};
+        uint16_t h[4], mi[4], lo[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = j0 + q;
+            float x = 0.0f;
+            if (r < cs && j < Pa) x = (l[q] > -INFINITY ? l[q] + logphi0 : 0.0f) - cm[q];
+            h[q] = f2bf(x);
+            const float r1 = x - bf2f(h[q]);
+            mi[q] = f2bf(r1);
+            lo[q] = f2bf(r1 - bf2f(mi[q]));
+        }
+        const size_t o = (size_t)r * Pp + j0;
+        *reinterpret_cast<uint2 *>(Lh + o) = pack(h);
+        if (NT == 2) {
+            *reinterpret_cast<uint2 *>(Ll + o) = pack(mi);
+        } else {
+            *reinterpret_cast<uint2 *>(Lm + o) = pack(mi);
+            *reinterpret_cast<uint2 *>(Ll + o) = pack(lo);
+        }
     }
 }
 
@@ -475,7 +486,7 @@ hipError_t launch_slice_f32(const emc_ctx *c, int s0, int cs, hipStream_t st)
 hipError_t launch_slice_bf16(const emc_ctx *c, int s0, int cs, hipStream_t st)
 {
     if (c->lw32) {
-        const dim3 g((c->Pp / 4 + 255) / 256, c->chunkp);
+        const dim3 g((c->Pp / 4 + 255) / 256, (c->chunkp + kSplitRows - 1) / kSplitRows);
         if (c->la_terms == 3)
             hipLaunchKernelGGL(k_split_bf16<3>, g, dim3(256), 0, st, c->lw32, s0, cs, c->Pa, c->Pp, c->colmean, c->scal,
                                c->Lh, c->Lm, c->Ll);
