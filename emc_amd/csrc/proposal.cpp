@@ -469,7 +469,7 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->full_every = 5;
     o->hidden = 512;
     o->epochs = 4;
-    o->batch = 256;
+    o->batch = 1024;  // 256 at Adam step 1e-3 trained 1.7x slower for no better capture
     o->synth_patterns = -1;
     o->refresh_epochs = 1;
     o->recall = 8;
@@ -477,7 +477,7 @@ extern "C" void emc_proposal_defaults(emc_proposal_options *o)
     o->refresh_fraction = 0.25f;
     o->train_below = 0.9999f;
     o->min_mass = 0.99f;
-    o->lr = 1e-3f;
+    o->lr = 1.5e-3f;
     o->max_fallback = 0.1f;
     o->seed = 1;
 }

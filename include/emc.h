@@ -321,7 +321,7 @@ typedef struct emc_proposal_options {
     int32_t full_every;     /* after warm-up, every full_every-th iteration searches fully (5) */
     int32_t hidden;         /* hidden width, multiple of 64 (512) */
     int32_t epochs;         /* passes over the training set after each full search (4) */
-    int32_t batch;          /* minibatch rows, multiple of 64 (256) */
+    int32_t batch;          /* minibatch rows, multiple of 64 (1024) */
     int32_t synth_patterns; /* model-sliced training patterns per pass; -1 = n_patterns / 4 */
     int32_t refresh_epochs; /* passes over fresh model-sliced patterns after each proposal
                                iteration's compress, so the network follows the model (1) */
@@ -335,7 +335,7 @@ typedef struct emc_proposal_options {
                                so the network learns the current model's orientation map rather
                                than the previous assignments (1) */
     float min_mass;        /* per-pattern predicted mass below which it is searched fully (0.99) */
-    float lr;               /* Adam step size (1e-3; beta 0.9 / 0.999, eps 1e-8) */
+    float lr;               /* Adam step size (1.5e-3; beta 0.9 / 0.999, eps 1e-8) */
     float max_fallback;     /* fraction of fully searched patterns above which the iteration
                                runs a full search instead (0.1) */
     float refresh_fraction; /* share of synth_patterns drawn for each refresh pass (0.25) */

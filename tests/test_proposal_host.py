@@ -51,7 +51,8 @@ def test_partition_rejects_bad_input():
 
 def test_proposal_defaults():
     o = emc_amd.proposal_options()
-    assert (o.top_k, o.warmup_full, o.full_every, o.hidden, o.batch) == (8, 3, 5, 512, 256)
+    assert (o.top_k, o.warmup_full, o.full_every, o.hidden, o.batch) == (8, 3, 5, 512, 1024)
+    assert abs(o.lr - 1.5e-3) < 1e-9
     assert abs(o.min_mass - 0.99) < 1e-6 and o.synth_patterns == -1
     assert (o.recall, o.refresh_epochs, o.train_patterns) == (8, 1, 1)
     assert abs(o.train_below - 0.9999) < 1e-7
