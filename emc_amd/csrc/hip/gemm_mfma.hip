@@ -491,9 +491,10 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
     }
 }
 
-// ---- logL contraction with DMA operand staging -----------------------------------------------
-// Two-term A (log W hi, lo) x one-term BF16 B (counts), 256 x 256 macro tile, 8 wavefronts as in
-// k_gemm_mfma2, BK2 = 32.  The operands go from global memory straight into LDS through the
+// ---- contractions with DMA operand staging ---------------------------------------------------
+// Two-term A (log W or P, hi + lo) x one-term BF16 B (counts), 256 x 256 macro tile, 8 wavefronts
+// as in k_gemm_mfma2, BK2 = 32; the logL epilogue (k_logl_dma) or the store epilogue (k_store_dma:
+// dense M-step, chi^2 sums).  The operands go from global memory straight into LDS through the
 // vector-memory DMA path (global_load_lds_dwordx4: lane l of a wave instruction lands at
 // M0 + 16 l), so no register staging and no LDS store instructions.  LDS rows are 64 B (32 BF16)
 // without padding; the four 16-byte chunks of row r sit at chunk ^ lds_swz(r), which spreads the
@@ -501,8 +502,8 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
 // of step k + 2 are issued during step k.  Each step reads the fragments of its second k-half
 // while the first half multiplies, and those of the next step's first half (after the barrier)
 // while the second half multiplies.
-// Measured at 25,856 x 20,224 x 3,200 against the register-staged k_gemm_mfma2: 5.70 vs 5.90 ms
-// and 5.86 vs 6.28 ms on two boxes, bit-identical logits and band partials.
+// Measured for the logL at 25,856 x 20,224 x 3,200 against the register-staged k_gemm_mfma2: 5.70
+// vs 5.90 ms and 5.86 vs 6.28 ms on two boxes, bit-identical logits and band partials.
 constexpr int kDmaStages = 3;
 
 __device__ __forceinline__ int lds_swz(int r) { return ((r >> 2) & 1) | ((((r >> 1) ^ (r >> 3)) & 1) << 1); }
@@ -510,9 +511,9 @@ __device__ __forceinline__ int lds_swz(int r) { return ((r >> 2) & 1) | ((((r >>
 // One DMA load of 16 bytes per lane into LDS at the wave-uniform byte address lds.  Written as
 // inline asm: with the builtin, the compiler's LDS-DMA tracking puts a vmcnt(0) in front of every
 // LDS read, which would wait for the loads of later stages too; the kernel waits with explicit
-// vmcnt counts instead.  The instruction has no register result.
-// M0 is a reserved register to the compiler; k_logl_dma's code uses it nowhere else (clang warns
-// about the clobber, which is silenced here).
+// vmcnt counts instead.  The instruction has no register result; the s_nop is the one wait state
+// required between writing M0 and an LDS DMA.  M0 is a reserved register to the compiler; the DMA
+// kernels' code uses it nowhere else (clang warns about the clobber, which is silenced here).
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void lds_dma16(const void *g, uint32_t lds)
