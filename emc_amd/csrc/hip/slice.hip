@@ -308,8 +308,16 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
 {
     __shared__ double red[16];
     const int tid = threadIdx.x;
-    double acc = 0.0;
-    for (int s = tid; s < S; s += 1024) acc += sumW[s];
+    // eight partial sums per thread (eight loads in flight), added in a fixed order
+    double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int s8 = tid;
+    for (; s8 + 7 * 1024 < S; s8 += 8 * 1024)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p[q] += sumW[s8 + q * 1024];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (s8 + q * 1024 < S) p[q] += sumW[s8 + q * 1024];
+    double acc = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     acc = wave_sum(acc);
     if ((tid & 63) == 0) red[tid >> 6] = acc;
     __syncthreads();
@@ -327,6 +335,7 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
         scal[SC_SUMW] = SW;
     }
     const double inv2s2 = 1.0 / (2.0 * (double)sigma * (double)sigma);
+#pragma unroll 4
     for (int s = tid; s < Sp; s += 1024) {
         float b = 0.0f, rs = 0.0f;
         if (s < S) {

@@ -90,8 +90,16 @@ __global__ __launch_bounds__(256) void k_sm_final(const float *__restrict__ pm, 
 __global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, int n, double *__restrict__ out)
 {
     __shared__ double red[16];
-    double a = 0.0;
-    for (int k = threadIdx.x; k < n; k += 1024) a += x[k];
+    // eight partial sums per thread (eight loads in flight), added in a fixed order
+    double p[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int k = threadIdx.x;
+    for (; k + 7 * 1024 < n; k += 8 * 1024)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p[q] += x[k + q * 1024];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (k + q * 1024 < n) p[q] += x[k + q * 1024];
+    double a = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
     a = wave_sum(a);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
