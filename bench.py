@@ -382,7 +382,7 @@ def main():
     ap.add_argument("--chi2-steps", type=int, default=2, help="timed iterations of the chi^2 + scaling legs")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-frac", type=int, default=16, help="oracle runs 1/N of the rotations")
+    ap.add_argument("--cpu-frac", type=int, default=8, help="oracle runs 1/N of the rotations (about 15 s on 16 cores)")
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
     ap.add_argument("--mfma-staging", type=int, default=0, choices=[0, 1],
                     help="logL contraction: 0 = DMA-staged operands (default), 1 = register-staged")
