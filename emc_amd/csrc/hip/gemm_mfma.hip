@@ -263,17 +263,32 @@ __device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
 // logL epilogue of the 256-row large-tile kernels (8 waves: 4 along M x 2 along N, each a 64-row
 // by BN2/2-column block of 2 x TN 32x32 tiles).  Accumulator layout: lane holds column lane % 32
 // of each 32x32 tile and rows 8*(r/4) + 4*(lane/32) + r%4, r = 0..15.  Stores R = acc + bias_s,
-// then (when e.pm is set) the softmax partials of this 256-row band, restating k_sm_partial's
-// online max / sum of exp with the same -1e10 floors: column maxima over the lane pair, then over
-// the 4 row-waves through LDS; then sums of exp(v - max) the same way.  lds: >= 12 BN2 bytes,
-// free (the main loop's stages).  kBiasLds: the tile's 256 row biases are read from bias_lds (the
-// kernel's LDS copy) instead of global memory, so no vector-memory wait holds up the logit stores
-// (on gfx9 a wait for a load also waits for every store issued before it).
-template <int TN, int BN2, bool kBiasLds = false>
-__device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm,
-                                              int wn, void *lds, const float *bias_lds = nullptr)
+// then the softmax partials of this 256-row band (e.pm and e.ps: the large-tile logL kernels run
+// only with the softmax fused), restating k_sm_partial's
+// rules: column maxima from the -1e10 floor, over the lane pair, then over the 4 row-waves through
+// LDS; then the sums of exp(R - max), where a term with R - max <= -1e10 (or not a number) adds
+// nothing.  Each lane sums its 32 terms of a column in FP32 on the hardware exp2 (the exponent
+// floored at -1e10, whose exp2 is 0); the lane and wave partials are added in FP64.  A tile inside
+// the valid rows and columns (every tile but the last row band and column tile) takes a path
+// without bounds checks that stores each row through a wave-uniform base address.  lds: 48 BN2
+// bytes, free (the main loop's stages).  bias_lds: the tile's 256 row biases, copied to LDS by the
+// kernel (0 past m_valid), so no vector-memory wait holds up the logit stores (on gfx9 a wait for
+// a load also waits for every store issued before it) and no bias stays in registers between the
+// two passes.
+template <int TN, int BN2, bool kFull>
+__device__ __forceinline__ void logl_epi(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm, int wn,
+                                         void *lds, const float *bias_lds)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    wm = __builtin_amdgcn_readfirstlane(wm);
+    wn = __builtin_amdgcn_readfirstlane(wn);
+    const int hr = 4 * (lane >> 5), cl = lane & 31;
+    const uint32_t lo = (uint32_t)(hr * e.ldc + cl);  // lane offset from a row's base
+    // bias of accumulator row r of tile i; -inf past m_valid (adds to neither the maximum nor the sum)
+    auto bias = [&](int i, int r) {
+        const int mt = wm + i * 32 + 8 * (r >> 2) + (r & 3) + hr;
+        return kFull || m0 + mt < e.m_valid ? bias_lds[mt] : -INFINITY;
+    };
     float cmax[TN];
 #pragma unroll
     for (int j = 0; j < TN; ++j) cmax[j] = -1.0e10f;
@@ -281,20 +296,30 @@ __device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const 
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (m >= e.m_valid) continue;
-            float add;
-            if constexpr (kBiasLds) add = bias_lds[m - m0];
-            else add = e.bias[m];
+            const int mt = wm + i * 32 + 8 * (r >> 2) + (r & 3);  // tile row of lanes 0-31
+            const float add = bias(i, r);
+            float *row = e.C + (size_t)(m0 + mt) * e.ldc + (n0 + wn);
 #pragma unroll
             for (int j = 0; j < TN; ++j) {
-                const int n = n0 + wn + j * 32 + (lane & 31);
                 const float v = acc[i][j][r] + add;
-                if (n < e.n_valid) e.C[(size_t)m * e.ldc + n] = v;
+                if constexpr (kFull) row[lo + j * 32] = v;
                 cmax[j] = fmaxf(cmax[j], v);
             }
         }
-    if (!e.pm) return;
+    if constexpr (!kFull) {  // stores of the valid rows and columns, in a loop of their own
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int mt = wm + i * 32 + 8 * (r >> 2) + (r & 3);
+                if (m0 + mt + hr >= e.m_valid) continue;
+                const float add = bias(i, r);
+                float *row = e.C + (size_t)(m0 + mt) * e.ldc + (n0 + wn);
+#pragma unroll
+                for (int j = 0; j < TN; ++j)
+                    if (n0 + wn + j * 32 + cl < e.n_valid) row[lo + j * 32] = acc[i][j][r] + add;
+            }
+    }
     float *red = reinterpret_cast<float *>(lds);            // [4][BN2] maxima
     double *reds = reinterpret_cast<double *>(lds) + 2 * BN2;  // [4][BN2] sums (after the maxima)
     const int wrow = wave >> 1;
@@ -306,42 +331,46 @@ __device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const 
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-        const int cl = wn + j * 32 + (lane & 31);
-        cmax[j] = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
+        const int c = wn + j * 32 + cl;
+        cmax[j] = fmaxf(fmaxf(red[c], red[BN2 + c]), fmaxf(red[2 * BN2 + c], red[3 * BN2 + c]));
     }
-    double csum[TN];
+    constexpr float kLog2e = 1.4426950408889634f;
+    float fs[TN];
 #pragma unroll
-    for (int j = 0; j < TN; ++j) csum[j] = 0.0;
+    for (int j = 0; j < TN; ++j) fs[j] = 0.0f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int m = m0 + wm + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            if (m >= e.m_valid) continue;
-            float add;
-            if constexpr (kBiasLds) add = bias_lds[m - m0];
-            else add = e.bias[m];
+            const float add = bias(i, r);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                const float d = (acc[i][j][r] + add) - cmax[j];
-                if (d > -1.0e10f) csum[j] += (double)expf(d);
-            }
+            for (int j = 0; j < TN; ++j)
+                fs[j] += __builtin_amdgcn_exp2f(fmaxf((acc[i][j][r] + add) - cmax[j], -1.0e10f) * kLog2e);
         }
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
-        csum[j] += __shfl_xor(csum[j], 32, 64);
-        if (lane < 32) reds[wrow * BN2 + wn + j * 32 + lane] = csum[j];
+        double s = (double)fs[j];
+        s += __shfl_xor(s, 32, 64);
+        if (lane < 32) reds[wrow * BN2 + wn + j * 32 + lane] = s;
     }
     __syncthreads();
-    for (int cl = tid; cl < BN2; cl += 512) {
-        const int n = n0 + cl;
+    for (int c = tid; c < BN2; c += 512) {
+        const int n = n0 + c;
         if (n >= e.n_valid) continue;
-        const float mx = fmaxf(fmaxf(red[cl], red[BN2 + cl]), fmaxf(red[2 * BN2 + cl], red[3 * BN2 + cl]));
-        const double sm = reds[cl] + reds[BN2 + cl] + reds[2 * BN2 + cl] + reds[3 * BN2 + cl];
+        const float mx = fmaxf(fmaxf(red[c], red[BN2 + c]), fmaxf(red[2 * BN2 + c], red[3 * BN2 + c]));
+        const double sm = reds[c] + reds[BN2 + c] + reds[2 * BN2 + c] + reds[3 * BN2 + c];
         const int64_t o = (e.band0 + m0 / BM2) * e.ldp + n;
         e.pm[o] = mx;
         e.ps[o] = sm;
     }
+}
+
+template <int TN, int BN2>
+__device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm, int wn,
+                                              void *lds, const float *bias_lds)
+{
+    if (m0 + BM2 <= e.m_valid && n0 + BN2 <= e.n_valid) logl_epi<TN, BN2, true>(acc, e, m0, n0, wm, wn, lds, bias_lds);
+    else logl_epi<TN, BN2, false>(acc, e, m0, n0, wm, wn, lds, bias_lds);
 }
 
 template <int NA, int EPI, int BN2, bool BU8>
@@ -471,7 +500,10 @@ __global__ __launch_bounds__(512) void k_gemm_mfma2(const bf16_t *__restrict__ A
     }
 
     if constexpr (EPI == EPI_LOGL) {
-        logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem);
+        float *bias_l = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(smem) + 48 * BN2);
+        if (tid < BM2) bias_l[tid] = m0 + tid < e.m_valid ? e.bias[m0 + tid] : 0.0f;
+        __syncthreads();
+        logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem, bias_l);
     } else {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -641,7 +673,7 @@ __device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, con
     mm(F1);
     __syncthreads();  // the epilogue reuses the LDS
     if constexpr (EPI == EPI_LOGL) {
-        logl_epilogue<TN, BN2, true>(acc, e, m0, n0, wm, wn, smem, bias_s);
+        logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem, bias_s);
     } else {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
