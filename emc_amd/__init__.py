@@ -300,7 +300,8 @@ class Engine:
                 "weight_sum": st.weight_sum, "stage_ms": {k: float(st.stage_ms[i]) for i, k in enumerate(STAGES)},
                 "search": SEARCH.get(st.search, st.search), "n_fallback": st.n_fallback,
                 "predicted_mass": st.predicted_mass, "captured_mass": st.captured_mass,
-                "train_loss": st.train_loss, "work_fraction": st.work_fraction}
+                "train_loss": st.train_loss, "work_fraction": st.work_fraction,
+                "logits_skipped": st.logits_skipped}
 
     # ---- learned orientation proposal (no reference counterpart) ----
 

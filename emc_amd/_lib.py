@@ -37,14 +37,15 @@ class Options(C.Structure):
                 ("device", C.c_int32), ("blur_sigma", C.c_double), ("fluence", C.c_double),
                 ("timing", C.c_int32), ("mfma_tile", C.c_int32), ("count_bytes", C.c_int32),
                 ("mstep", C.c_int32), ("scaling", C.c_int32), ("tile_rows", C.c_int32),
-                ("mfma_staging", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("mfma_staging", C.c_int32), ("keep_logits", C.c_int32)]
 
 
 class IterStats(C.Structure):
     _fields_ = [("total_logl", C.c_double), ("sigma", C.c_double), ("fluence", C.c_double),
                 ("model_avg", C.c_double), ("weight_sum", C.c_double), ("stage_ms", C.c_float * N_STAGES),
                 ("search", C.c_int32), ("n_fallback", C.c_int32), ("predicted_mass", C.c_double),
-                ("captured_mass", C.c_double), ("train_loss", C.c_double), ("work_fraction", C.c_double)]
+                ("captured_mass", C.c_double), ("train_loss", C.c_double), ("work_fraction", C.c_double),
+                ("logits_skipped", C.c_double)]
 
 
 class ProposalOptions(C.Structure):

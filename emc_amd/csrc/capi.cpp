@@ -200,7 +200,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colp); dfree(c->colpart); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
     dfree(c->Cx); dfree(c->K2h); dfree(c->K2l); dfree(c->Aw); dfree(c->G2); dfree(c->G3); dfree(c->G4);
-    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->pdirty); dfree(c->lse); dfree(c->part); dfree(c->red); dfree(c->T); dfree(c->U); dfree(c->scal);
+    dfree(c->R); dfree(c->Ph); dfree(c->Pl); dfree(c->pdirty); dfree(c->runmax); dfree(c->blk); dfree(c->lse); dfree(c->part); dfree(c->red); dfree(c->T); dfree(c->U); dfree(c->scal);
     dfree(c->bin_vals); dfree(c->bin_perm); dfree(c->bin_vbeg); dfree(c->bin_frac); dfree(c->bin_units); dfree(c->bin_blocks); dfree(c->bin_groups);
     dfree(c->bin_gbeg); dfree(c->bin_part); dfree(c->bin_gpart);
     if (c->scal_h) (void)hipHostFree(c->scal_h);
@@ -281,6 +281,8 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
         if (st == EMC_OK) A(dalloc(&c->Pl, Sp * Ip), "Pl");
         if (st == EMC_OK) A(dalloc(&c->pdirty, Sp * (int64_t)p_segments(c)), "pdirty");
         if (st == EMC_OK) A(hipMemset(c->pdirty, 1, sizeof(uint8_t) * Sp * p_segments(c)), "pdirty");
+        if (st == EMC_OK) A(dalloc(&c->runmax, Ip), "runmax");
+        if (st == EMC_OK) A(dalloc(&c->blk, (int64_t)softmax_fused_bands(c) * Ip), "blk");
         if (st == EMC_OK) A(dalloc(&c->colsum, 2 * (int64_t)PpMax), "colsum");
         if (st == EMC_OK) A(dalloc(&c->colp, colp_len(c->S, PpMax)), "colp");
         if (st == EMC_OK) A(dalloc(&c->colpart, colpart_len(PpMax)), "colpart");
@@ -605,7 +607,8 @@ extern "C" emc_status emc_get_logl_block(emc_ctx *c, int32_t s0, int32_t ns, int
     if (!c || !logits) return fail(c, EMC_E_INVALID, "get_logl: null argument");
     if (bad_block(c, s0, ns, i0, ni)) return fail(c, EMC_E_INVALID, "get_logl: block out of range");
     if (!c->has_logl) return fail(c, EMC_E_STATE, "get_logl: logits not available (run emc_logl; the FP32 path "
-                                                  "overwrites them in emc_normalize_resp)");
+                                                  "overwrites them in emc_normalize_resp; emc_iterate keeps them "
+                                                  "all only with options.keep_logits = 1)");
     if (ns == 0 || ni == 0) return EMC_OK;
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -737,6 +740,7 @@ extern "C" emc_status emc_logl(emc_ctx *c, float sigma)
     if (c->opt.diff != EMC_DIFF_TRUE_POISSON && !(sigma > 0.0f)) return fail(c, EMC_E_INVALID, "logl: sigma must be > 0");
     HIP_TRY(c, hipSetDevice(c->opt.device));
     HIP_TRY(c, launch_fill_f64(c->scal + SC_NONFINITE, 0.0, 1, c->stream));
+    c->skip_logits = c->logits_partial = false;  // the caller may read every logit
     EMC_TRY(logl_enqueue(c, sigma));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->has_logl = true;
@@ -1004,6 +1008,10 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         }
     }
     if (full) {
+        // negligible bands' logit stores are left out unless the caller keeps the logits or the
+        // proposal's recall reads them
+        c->skip_logits = !c->opt.keep_logits && !(pr && pr->o.recall > 0);
+        c->logits_partial = logl_skips_stores(c);
         if (sliced) {  // one chunk covers every rotation while the proposal is enabled
             HIP_TRY(c, launch_logl_mfma(c, 0, c->S, c->stream));
             gev_rec(c, 0, 0, 1);
@@ -1011,6 +1019,7 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         } else {
             EMC_TRY(logl_enqueue(c, (float)sigma));
         }
+        c->skip_logits = false;
         EMC_TRY(softmax_enqueue(c));
         EMC_TRY(mstep_enqueue(c));
         if (pr) {
@@ -1044,7 +1053,7 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
     if (c->opt.timing) (void)hipEventRecord(c->ev[2 * EMC_STAGE_TOTAL + 1], c->stream);
     EMC_TRY(sync_scalars(c));
     if (pr) ++pr->iter;
-    c->has_logl = full && c->mfma;
+    c->has_logl = full && c->mfma && !c->logits_partial;
     c->has_resp = full;  // a proposal iteration leaves per-cell blocks, not the [S][I] matrix
     c->has_lse = true;   // lse and the slice sums T of either search
     c->has_mstep = true;
@@ -1062,6 +1071,8 @@ extern "C" emc_status emc_iterate(emc_ctx *c, int32_t iteration, emc_iter_stats 
         stats->captured_mass = captured;
         stats->train_loss = train_loss;
         stats->work_fraction = full ? 1.0 : pr->work_fraction;
+        if (full && c->logits_partial)
+            stats->logits_skipped = c->scal_h[SC_SKIPPED] / ((double)softmax_fused_bands(c) * c->I);
         if (c->opt.timing) {
             const int stages[] = {EMC_STAGE_SLICE, EMC_STAGE_LOGL, EMC_STAGE_SOFTMAX, EMC_STAGE_MSTEP, EMC_STAGE_COMPRESS,
                                   EMC_STAGE_TOTAL};

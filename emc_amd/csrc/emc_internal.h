@@ -36,6 +36,7 @@ enum {
     SC_B0 = 10,       // global logit shift removed from the stored logits (true Poisson)
     SC_LOSS = 11,     // proposal network: summed cross-entropy of a training pass
     SC_NNZ = 12,      // non-zero responsibilities of the last dense E-step (BF16 hi or lo non-zero)
+    SC_SKIPPED = 13,  // (band, pattern) logit blocks whose stores the logL kernel left out
     SC_N = 16
 };
 
@@ -182,6 +183,12 @@ struct emc_ctx {
     float *R = nullptr;                                // [Sp][Ip] logits (P in place on FP32 path)
     emc::bf16_t *Ph = nullptr, *Pl = nullptr;          // [Sp][Ip] BF16 hi/lo P (MFMA path)
     uint8_t *pdirty = nullptr;  // [Sp][ceil(I/256)]: 0 = that 256-pattern segment of Ph/Pl holds only zeros
+    int32_t *runmax = nullptr;  // [Ip] per pattern, the largest band maximum of the logits found so far in
+                                // this logL pass (order-preserving integer form of a float)
+    int16_t *blk = nullptr;     // [fused bands][Ip] where the skipping logL pass put each band's logits of
+                                // a pattern: block row (>= 0), -2 in place, -1 not stored
+    bool skip_logits = false;   // the current logL pass may leave out negligible bands' logit stores
+    bool logits_partial = false; // the last logL pass left some out
     double *lse = nullptr;                             // [Ip] lse' (constants excluded)
     double *part = nullptr;                            // softmax / reduction partials
     double *red = nullptr;                             // [kRedLen] model-statistics partials (own buffer:
@@ -192,12 +199,12 @@ struct emc_ctx {
     double *scal = nullptr;                            // [SC_N] device scalars
     double *scal_h = nullptr;                          // pinned host mirror
     // binned insertion: valid (s, j) pairs sorted by (chunk, model brick), cut into units of at most
-    // 2,048 pairs; per unit a base-voxel permutation; partial halo bricks per block (<= 4 units of a
+    // 4,096 pairs; per unit a base-voxel permutation; partial halo bricks per block (<= 4 units of a
     // brick, one workgroup) and per group of blocks
     uint32_t *bin_vals = nullptr;                      // [S * Pa] packed s << jbits | j, (s, j) order per brick
     uint16_t *bin_perm = nullptr;                      // [S * Pa] each pair's position in base-voxel order in its unit
     uint16_t *bin_vbeg = nullptr;                      // [n_units][520] start of each base voxel's run
-    float *bin_frac = nullptr;                         // [n_units][3][2048] trilinear fractions (base-voxel order)
+    float *bin_frac = nullptr;                         // [n_units][3][4096] trilinear fractions (base-voxel order)
     int32_t *bin_units = nullptr;                      // [n_units][4] = key, begin, end, 0
     int32_t *bin_blocks = nullptr;                     // [n_blocks][4] = key, first unit, end unit, 0 (one workgroup)
     int32_t *bin_groups = nullptr;                     // [n_groups][4] = key, first block, end block (per chunk), 0
@@ -217,6 +224,7 @@ hipError_t launch_rotmat(const float *q, const float *w, int S, int Sp, float *r
                          hipStream_t st);
 hipError_t launch_fill_f32(float *p, float v, int64_t n, hipStream_t st);
 hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st);
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st);
 
 // Slicing statistics over rotations [0, S): sumW, ns and (if colsum) sum of log W per pixel.
 // colp / colpart hold colp_len / colpart_len elements.
@@ -274,6 +282,8 @@ hipError_t launch_mstep_sparse(const emc_ctx *c, int s0, int cs, const double *g
 // per-band softmax partials (one band per 256 rotation rows) and launch_softmax skips its pass.
 bool logl_fuses_softmax(const emc_ctx *c);
 int softmax_fused_bands(const emc_ctx *c);
+// The logL pass leaves out the stores of negligible bands' logits (c->skip_logits on the DMA kernel).
+bool logl_skips_stores(const emc_ctx *c);
 int64_t softmax_part_len(int S, int I, int fused_bands);
 hipError_t launch_softmax(const emc_ctx *c, hipStream_t st);
 // Atomic insertion of U rows [s0, s0+cs).

@@ -47,6 +47,11 @@ struct EpiArgs {
     float *pm;
     double *ps;
     int64_t band0, ldp;
+    // the DMA logL kernel's store skipping (null: every logit is stored): per pattern the running
+    // maximum of the band maxima (ord_of form), and the count of columns left out
+    int32_t *runmax;
+    double *skipped;
+    int16_t *blk;  // [band][ldp] where each column's logits went (null: every logit in place)
     // device-side choice between this kernel and the sparse M-step: the whole grid returns at
     // once when *gate <= gate_thr (the responsibilities' non-zero count is small)
     const double *gate;
@@ -260,24 +265,48 @@ __device__ __forceinline__ u32x4 widen_u8x8(u32x2 v)
     return o;
 }
 
+// Order-preserving integer form of a float (signed compare = float compare, NaN aside), for the
+// running per-pattern maxima kept with integer atomicMax; kOrdNegInf is the form of -inf.
+__device__ __forceinline__ int32_t ord_of(float x)
+{
+    const int32_t b = __float_as_int(x);
+    return b >= 0 ? b : b ^ 0x7fffffff;
+}
+__device__ __forceinline__ float float_of_ord(int32_t k) { return __int_as_float(k >= 0 ? k : k ^ 0x7fffffff); }
+constexpr int32_t kOrdNegInf = (int32_t)0x807fffff;
+
+// A band's logits of a pattern are not stored when the band's largest one lies more than this many
+// nats below a largest logit already found for the pattern: every responsibility in the band is then
+// below exp(-128) of the pattern's largest one, and the softmax, which treats every band more than
+// kSkipApplyNats below the pattern's lse as -inf, computes P = 0 there from the band maxima alone.
+constexpr double kSkipStoreNats = 128.0;
+
 // logL epilogue of the 256-row large-tile kernels (8 waves: 4 along M x 2 along N, each a 64-row
 // by BN2/2-column block of 2 x TN 32x32 tiles).  Accumulator layout: lane holds column lane % 32
-// of each 32x32 tile and rows 8*(r/4) + 4*(lane/32) + r%4, r = 0..15.  Stores R = acc + bias_s,
-// then the softmax partials of this 256-row band (e.pm and e.ps: the large-tile logL kernels run
-// only with the softmax fused), restating k_sm_partial's
-// rules: column maxima from the -1e10 floor, over the lane pair, then over the 4 row-waves through
-// LDS; then the sums of exp(R - max), where a term with R - max <= -1e10 (or not a number) adds
-// nothing.  Each lane sums its 32 terms of a column in FP32 on the hardware exp2 (the exponent
-// floored at -1e10, whose exp2 is 0); the lane and wave partials are added in FP64.  A tile inside
-// the valid rows and columns (every tile but the last row band and column tile) takes a path
-// without bounds checks that stores each row through a wave-uniform base address.  lds: 48 BN2
-// bytes, free (the main loop's stages).  bias_lds: the tile's 256 row biases, copied to LDS by the
-// kernel (0 past m_valid), so no vector-memory wait holds up the logit stores (on gfx9 a wait for
-// a load also waits for every store issued before it) and no bias stays in registers between the
-// two passes.
+// of each 32x32 tile and rows 8*(r/4) + 4*(lane/32) + r%4, r = 0..15.  R = acc + bias_s.  The
+// softmax partials of this 256-row band (e.pm and e.ps: the large-tile logL kernels run only with
+// the softmax fused) restate k_sm_partial's rules: column maxima from the -1e10 floor, over the lane
+// pair, then over the 4 row-waves through LDS; then the sums of exp(R - max), where a term with
+// R - max <= -1e10 (or not a number) adds nothing.  Each lane sums its 32 terms of a column in FP32
+// on the hardware exp2 (the exponent floored at -1e10, whose exp2 is 0); the lane and wave partials
+// are added in FP64.
+// The logits are stored after the maxima are known.  Without e.blk every logit is stored in place
+// (row-major R).  With e.blk (the iteration's store skipping): a column whose band maximum lies
+// kSkipStoreNats below rmk[column of the tile] (running maxima read when the workgroup started:
+// lower bounds of the pattern's largest logit) is not stored; in a tile inside the valid rows and
+// columns the kept columns are packed, in column order, as blocks of 256 consecutive floats (the
+// column's rows) in the tile's own rows of R (block k in row m0 + k, columns n0 .. n0 + 255), so the
+// HBM writes are whole lines however few columns are kept; the other tiles store their kept columns
+// in place.  e.blk[band][pattern] records the block (>= 0), -2 (in place) or -1 (not stored); every
+// band maximum is merged into e.runmax, and e.skipped counts the columns left out.
+// A tile inside the valid rows and columns (every tile but the last row band and column tile) takes
+// a path without bounds checks.  lds: 48 BN2 + 1056 bytes, free (the main loop's stages).  bias_lds:
+// the tile's 256 row biases, copied to LDS by the kernel (0 past m_valid), so no vector-memory wait
+// holds up the logit stores (on gfx9 a wait for a load also waits for every store issued before it)
+// and no bias stays in registers between the passes.
 template <int TN, int BN2, bool kFull>
 __device__ __forceinline__ void logl_epi(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm, int wn,
-                                         void *lds, const float *bias_lds)
+                                         void *lds, const float *bias_lds, const int32_t *rmk)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     wm = __builtin_amdgcn_readfirstlane(wm);
@@ -296,43 +325,41 @@ __device__ __forceinline__ void logl_epi(const f32x16 (&acc)[2][TN], const EpiAr
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-            const int mt = wm + i * 32 + 8 * (r >> 2) + (r & 3);  // tile row of lanes 0-31
             const float add = bias(i, r);
-            float *row = e.C + (size_t)(m0 + mt) * e.ldc + (n0 + wn);
 #pragma unroll
-            for (int j = 0; j < TN; ++j) {
-                const float v = acc[i][j][r] + add;
-                if constexpr (kFull) row[lo + j * 32] = v;
-                cmax[j] = fmaxf(cmax[j], v);
-            }
+            for (int j = 0; j < TN; ++j) cmax[j] = fmaxf(cmax[j], acc[i][j][r] + add);
         }
-    if constexpr (!kFull) {  // stores of the valid rows and columns, in a loop of their own
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int mt = wm + i * 32 + 8 * (r >> 2) + (r & 3);
-                if (m0 + mt + hr >= e.m_valid) continue;
-                const float add = bias(i, r);
-                float *row = e.C + (size_t)(m0 + mt) * e.ldc + (n0 + wn);
-#pragma unroll
-                for (int j = 0; j < TN; ++j)
-                    if (n0 + wn + j * 32 + cl < e.n_valid) row[lo + j * 32] = acc[i][j][r] + add;
-            }
-    }
     float *red = reinterpret_cast<float *>(lds);            // [4][BN2] maxima
     double *reds = reinterpret_cast<double *>(lds) + 2 * BN2;  // [4][BN2] sums (after the maxima)
-    const int wrow = wave >> 1;
+    int *cnt = reinterpret_cast<int *>(reinterpret_cast<unsigned char *>(lds) + 48 * BN2 + 1024);  // [2 TN]
+    const int wrow = wave >> 1, half = wn / (BN2 / 2);
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
         cmax[j] = fmaxf(cmax[j], __shfl_xor(cmax[j], 32, 64));
         if (lane < 32) red[wrow * BN2 + wn + j * 32 + lane] = cmax[j];
     }
     __syncthreads();
+    bool keep[TN];
 #pragma unroll
     for (int j = 0; j < TN; ++j) {
         const int c = wn + j * 32 + cl;
         cmax[j] = fmaxf(fmaxf(red[c], red[BN2 + c]), fmaxf(red[2 * BN2 + c], red[3 * BN2 + c]));
+        keep[j] = !(rmk && (double)cmax[j] < (double)float_of_ord(rmk[c]) - kSkipStoreNats);
+        if (!kFull) keep[j] = keep[j] && n0 + c < e.n_valid;
+    }
+    const bool compact = kFull && e.blk && BN2 == 256;  // a block is 256 floats of the tile's row
+    // kept columns of each group of 32 (the same in every row-wave: keep depends on the column only)
+    uint32_t km[TN];
+#pragma unroll
+    for (int j = 0; j < TN; ++j) km[j] = (uint32_t)__ballot(keep[j]);
+    if (compact && wrow == 0 && lane == 0)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) cnt[half * TN + j] = __popc(km[j]);
+    if (rmk && wrow == 0) {  // the columns left out, counted once (row-wave 0, lanes 0-31)
+        int nskip = 0;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) nskip += __popc(~km[j] & (uint32_t)__ballot(n0 + wn + j * 32 + cl < e.n_valid));
+        if (lane == 0 && nskip > 0) atomicAdd(e.skipped, (double)nskip);
     }
     constexpr float kLog2e = 1.4426950408889634f;
     float fs[TN];
@@ -353,24 +380,64 @@ __device__ __forceinline__ void logl_epi(const f32x16 (&acc)[2][TN], const EpiAr
         s += __shfl_xor(s, 32, 64);
         if (lane < 32) reds[wrow * BN2 + wn + j * 32 + lane] = s;
     }
-    __syncthreads();
+    __syncthreads();  // also publishes the kept-column counts
+    const int64_t orow = (e.band0 + m0 / BM2) * e.ldp;  // this band's row of pm, ps and blk
+    if (compact) {
+        int off = 0;  // blocks of the groups before (half, j) in column order
+#pragma unroll
+        for (int g = 0; g < 2 * TN; ++g) off += g < half * TN ? cnt[g] : 0;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int rank = off + __popc(km[j] & ((1u << cl) - 1u));
+            off += __popc(km[j]);
+            if (wrow == 0 && lane < 32) e.blk[orow + n0 + wn + j * 32 + cl] = (int16_t)(keep[j] ? rank : -1);
+            if (!keep[j]) continue;
+            float *blk = e.C + (size_t)(m0 + rank) * e.ldc + n0;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {  // rows 8 g + hr + 0..3 of tile i: 16 contiguous bytes
+                    const int mr = wm + i * 32 + 8 * g + hr;
+                    const float4 v = make_float4(acc[i][j][4 * g] + bias(i, 4 * g), acc[i][j][4 * g + 1] + bias(i, 4 * g + 1),
+                                                 acc[i][j][4 * g + 2] + bias(i, 4 * g + 2),
+                                                 acc[i][j][4 * g + 3] + bias(i, 4 * g + 3));
+                    *reinterpret_cast<float4 *>(blk + mr) = v;
+                }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            if (e.blk && wrow == 0 && lane < 32 && n0 + wn + j * 32 + cl < e.n_valid)
+                e.blk[orow + n0 + wn + j * 32 + cl] = (int16_t)(keep[j] ? -2 : -1);
+            if (!keep[j]) continue;
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int mt = wm + i * 32 + 8 * (r >> 2) + (r & 3);  // tile row of lanes 0-31
+                    if (!kFull && m0 + mt + hr >= e.m_valid) continue;
+                    float *row = e.C + (size_t)(m0 + mt) * e.ldc + (n0 + wn);
+                    row[lo + j * 32] = acc[i][j][r] + bias(i, r);
+                }
+        }
+    }
     for (int c = tid; c < BN2; c += 512) {
         const int n = n0 + c;
         if (n >= e.n_valid) continue;
         const float mx = fmaxf(fmaxf(red[c], red[BN2 + c]), fmaxf(red[2 * BN2 + c], red[3 * BN2 + c]));
         const double sm = reds[c] + reds[BN2 + c] + reds[2 * BN2 + c] + reds[3 * BN2 + c];
-        const int64_t o = (e.band0 + m0 / BM2) * e.ldp + n;
-        e.pm[o] = mx;
-        e.ps[o] = sm;
+        e.pm[orow + n] = mx;
+        e.ps[orow + n] = sm;
+        if (e.runmax && mx == mx) atomicMax(e.runmax + n, ord_of(mx));
     }
 }
 
 template <int TN, int BN2>
 __device__ __forceinline__ void logl_epilogue(const f32x16 (&acc)[2][TN], const EpiArgs &e, int m0, int n0, int wm, int wn,
-                                              void *lds, const float *bias_lds)
+                                              void *lds, const float *bias_lds, const int32_t *rmk = nullptr)
 {
-    if (m0 + BM2 <= e.m_valid && n0 + BN2 <= e.n_valid) logl_epi<TN, BN2, true>(acc, e, m0, n0, wm, wn, lds, bias_lds);
-    else logl_epi<TN, BN2, false>(acc, e, m0, n0, wm, wn, lds, bias_lds);
+    if (m0 + BM2 <= e.m_valid && n0 + BN2 <= e.n_valid) logl_epi<TN, BN2, true>(acc, e, m0, n0, wm, wn, lds, bias_lds, rmk);
+    else logl_epi<TN, BN2, false>(acc, e, m0, n0, wm, wn, lds, bias_lds, rmk);
 }
 
 template <int NA, int EPI, int BN2, bool BU8>
@@ -568,6 +635,7 @@ __device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, con
     constexpr uint32_t kVm0 = 0x0F70;         // s_waitcnt encoding: vmcnt(0), lgkmcnt and expcnt not waited
     __shared__ __attribute__((aligned(1024))) unsigned char smem[kDmaStages * STAGEB];
     __shared__ float bias_s[BM2];
+    __shared__ int32_t rm_s[BN2];
 
     // tile order as in k_gemm_mfma2
     const int nwg = gridDim.x, b = blockIdx.x;
@@ -638,11 +706,15 @@ __device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, con
     const int nk = Kd / BK2;
     Frag F0, F1;
     const float bv = EPI == EPI_LOGL && tid < BM2 && m0 + tid < e.m_valid ? e.bias[m0 + tid] : 0.0f;
+    // running maxima of the tile's columns for the store skipping, read now (kept in LDS): a value
+    // read earlier is still a lower bound of the pattern's largest logit when the epilogue uses it
+    const int32_t rv = EPI == EPI_LOGL && e.runmax && tid < BN2 && n0 + tid < e.n_valid ? e.runmax[n0 + tid] : kOrdNegInf;
     issue(0, 0);
     if (nk > 1) issue(1, 1);
     if (nk > 1) __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
     else __builtin_amdgcn_s_waitcnt(kVm0);
     if (tid < BM2) bias_s[tid] = bv;
+    if (tid < BN2) rm_s[tid] = rv;
     __syncthreads();
     rd(F0, 0, 0);
     int k = 0;
@@ -673,7 +745,7 @@ __device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, con
     mm(F1);
     __syncthreads();  // the epilogue reuses the LDS
     if constexpr (EPI == EPI_LOGL) {
-        logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem, bias_s);
+        logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem, bias_s, e.runmax ? rm_s : nullptr);
     } else {
 #pragma unroll
         for (int i = 0; i < 2; ++i)
@@ -776,6 +848,13 @@ bool logl_fuses_softmax(const emc_ctx *c) { return c->mfma && !c->chi2_mfma && !
 
 int softmax_fused_bands(const emc_ctx *c) { return (int)((int64_t)((c->S + c->chunk - 1) / c->chunk) * (c->chunkp / BM2)); }
 
+bool logl_skips_stores(const emc_ctx *c)
+{
+    // the softmax's 8-row groups must not straddle a band (chunks of a multiple of 8 rotations)
+    return c->skip_logits && logl_fuses_softmax(c) && c->la_terms == 2 && c->opt.mfma_staging == 0 &&
+           c->chunk % 8 == 0;
+}
+
 hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
 {
     EpiArgs e{};
@@ -795,6 +874,17 @@ hipError_t launch_logl_mfma(const emc_ctx *c, int s0, int cs, hipStream_t st)
         e.ps = c->part + nb * c->I;
         e.band0 = (int64_t)(s0 / c->chunk) * (c->chunkp / BM2);
         e.ldp = c->I;
+        if (logl_skips_stores(c)) {
+            // the running maxima start at -inf with the pass's first chunk and carry over the others
+            e.runmax = c->runmax;
+            e.skipped = c->scal + SC_SKIPPED;
+            e.blk = c->blk;
+            if (s0 == 0) {
+                hipError_t err = launch_fill_i32(c->runmax, kOrdNegInf, c->I, st);
+                if (err == hipSuccess) err = launch_fill_f64(e.skipped, 0.0, 1, st);
+                if (err != hipSuccess) return err;
+            }
+        }
         if (c->la_terms == 2 && c->opt.mfma_staging == 0)
             return run_dma<EPI_LOGL>(c->chunkp, c->Ip, st, c->Lh, A1, c->Pp, c->K16, c->K8, c->Pp, c->Pp, e);
         return run_large<EPI_LOGL>(c->chunkp, c->Ip, st, c->la_terms, c->Lh, A1, A2, c->Pp, c->K16, c->K8, c->Pp,

@@ -424,6 +424,14 @@ hipError_t launch_fill_f64(double *p, double v, int64_t n, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<int32_t>, dim3((unsigned)blocks), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
 int64_t colp_len(int S, int Pp) { return (int64_t)((S + kStatsG - 1) / kStatsG) * Pp; }
 int64_t colpart_len(int Pp) { return (int64_t)kColParts * Pp; }
 

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(1024) void k_sum_f64(const double *__restrict__ x, 
     }
 }
 
-constexpr int kApplyRows = 8;    // rotation rows per workgroup
+constexpr int kApplyRows = 8;    // rotation rows per workgroup (the block reads of k_sm_apply assume 8)
 
 // Two floats to a packed BF16 pair (a in the low half), round to nearest even: gfx950's
 // two-source v_cvt_pk_bf16_f32, which rounds as f2bf_dev does.  Written as asm because the compiler
@@ -128,12 +128,22 @@ constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
 // OUT = 1 skips the store of a wave's 256-pattern row segment when every value in it is zero and
 // pdirty says the segment already holds only zeros (P is mostly zeros once the posteriors are
 // sharp); pdirty is then updated to whether the segment holds a non-zero value.
+// pm (the logL epilogue's band maxima, [band][I], band of row s = (s / chunk) bpc + (s % chunk) / 256;
+// null without them): a pattern's logits in a band whose maximum lies more than kSkipApplyNats below
+// its lse are not read and count as -inf, which gives the P = 0 that they give (exp of less than -110
+// is 0 in FP32).  blk (the skipping logL pass's record, [band][I]; null when every logit is in
+// place): -1 = the band's logits of the pattern were not stored (they are negligible: the pass
+// leaves out only bands more than 128 below a maximum of the pattern, so pm marks them too), -2 = in
+// place, k >= 0 = the 256 logits of the band are the row brow0 + k of R from the pattern's 256-column
+// tile on (brow0 = the band's first row).
+constexpr double kSkipApplyNats = 110.0;
 template <int OUT>
 __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t ldr, int S, int I,
                                                   const double *__restrict__ lse, bf16_t *__restrict__ Ph,
                                                   bf16_t *__restrict__ Pl, double *__restrict__ Tpart,
                                                   double *__restrict__ nnz, const float *__restrict__ iphi,
-                                                  uint8_t *__restrict__ pdirty, int nseg)
+                                                  uint8_t *__restrict__ pdirty, int nseg, const float *__restrict__ pm,
+                                                  const int16_t *__restrict__ blk, int chunk, int bpc)
 {
     int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
     __shared__ double red[4][kApplyRows];
@@ -145,26 +155,64 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
     // of 2, Sterbenz), so exp((x - lh) - ll) carries the double-precision argument to float rounding
     // (a column past I gets lh = +inf: its p is exp2(-inf) = 0 without a select per element)
     float lh[4], ll[4];
+    bool neg[4];  // this band's logits of pattern c0 + k are negligible (not read)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const double l = (c0 + k < I) ? lse[c0 + k] : 0.0;
         lh[k] = (c0 + k < I) ? (float)l : INFINITY;
         ll[k] = (c0 + k < I) ? (float)(l - (double)lh[k]) : 0.0f;
+        neg[k] = false;
+    }
+    int code[4] = {-2, -2, -2, -2};
+    int brow0 = 0;
+    if (OUT == 1 && pm) {
+        const int rl = min(r0 + kApplyRows - 1, S - 1);
+        const int b0 = (r0 / chunk) * bpc + (r0 % chunk) / 256, b1 = (rl / chunk) * bpc + (rl % chunk) / 256;
+        if (b0 == b1) {  // the workgroup's rows lie in one band (always, when blk is given)
+            brow0 = (r0 / chunk) * chunk + ((r0 % chunk) / 256) * 256;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (c0 + k < I) {
+                    neg[k] = (double)pm[(size_t)b0 * I + c0 + k] < lse[c0 + k] - kSkipApplyNats;
+                    if (blk) code[k] = blk[(size_t)b0 * I + c0 + k];
+                    neg[k] = neg[k] || code[k] == -1;
+                }
+        }
     }
     // every row's logits and segment flag are loaded before the first is used, so the wave waits for
     // memory once rather than once per row
     // (rows past S read row S - 1 and a segment past the last reads the last; their values are not
     // used)
-    float4 vin[kApplyRows];
+    float xv[kApplyRows][4];
     int flag[kApplyRows];
     const int segc = OUT == 1 ? min(seg, nseg - 1) : 0;
+    const bool in_place = code[0] == -2 && code[1] == -2 && code[2] == -2 && code[3] == -2;
+    const bool read = !(neg[0] && neg[1] && neg[2] && neg[3]);
 #pragma unroll
     for (int rr = 0; rr < kApplyRows; ++rr) {
         const size_t sc = (size_t)min(r0 + rr, S - 1);
-        vin[rr] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (c0 < I) vin[rr] = *reinterpret_cast<const float4 *>(R + sc * ldr + c0);
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (c0 < I && read && in_place) v = *reinterpret_cast<const float4 *>(R + sc * ldr + c0);
+        xv[rr][0] = v.x;
+        xv[rr][1] = v.y;
+        xv[rr][2] = v.z;
+        xv[rr][3] = v.w;
         flag[rr] = OUT == 1 ? (int)pdirty[sc * nseg + segc] : 0;
     }
+    if (!in_place)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (neg[k]) continue;
+            if (code[k] >= 0) {  // 8 consecutive floats of the pattern's block
+                const float *b = R + (size_t)(brow0 + code[k]) * ldr + ((c0 + k) & ~255) + (r0 - brow0);
+                const float4 a0 = *reinterpret_cast<const float4 *>(b), a1 = *reinterpret_cast<const float4 *>(b + 4);
+                xv[0][k] = a0.x; xv[1][k] = a0.y; xv[2][k] = a0.z; xv[3][k] = a0.w;
+                xv[4][k] = a1.x; xv[5][k] = a1.y; xv[6][k] = a1.z; xv[7][k] = a1.w;
+            } else {
+#pragma unroll
+                for (int rr = 0; rr < kApplyRows; ++rr) xv[rr][k] = R[(size_t)min(r0 + rr, S - 1) * ldr + c0 + k];
+            }
+        }
     double rowsum[kApplyRows];
 #pragma unroll
     for (int rr = 0; rr < kApplyRows; ++rr) {
@@ -174,8 +222,8 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         int any_nz = 0;
         if (s < S && c0 < I) {
             float *src = R + (size_t)s * ldr + c0;
-            const float4 v = vin[rr];
-            const float x[4] = {v.x, v.y, v.z, v.w};
+            const float x[4] = {neg[0] ? -INFINITY : xv[rr][0], neg[1] ? -INFINITY : xv[rr][1],
+                                neg[2] ? -INFINITY : xv[rr][2], neg[3] ? -INFINITY : xv[rr][3]};
             // exp(d) = 2^(d log2 e) on the hardware exp2: relative error about |d| 6e-8, below the
             // 2^-16 of the BF16 pair P is stored as
             constexpr float kLog2e = 1.4426950408889634f;
@@ -314,10 +362,12 @@ hipError_t launch_softmax(const emc_ctx *c, hipStream_t st)
     double *nzpart = Tpart + (int64_t)ncb * S;  // one non-zero count per apply workgroup
     if (c->mfma)
         hipLaunchKernelGGL(k_sm_apply<1>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, c->Ph, c->Pl,
-                           Tpart, nzpart, c->opt.scaling ? c->iphi : nullptr, c->pdirty, p_segments(c));
+                           Tpart, nzpart, c->opt.scaling ? c->iphi : nullptr, c->pdirty, p_segments(c),
+                           fused ? pm : nullptr, fused && c->logits_partial ? c->blk : nullptr, c->chunk,
+                           c->chunkp / 256);
     else
         hipLaunchKernelGGL(k_sm_apply<0>, ga, dim3(256), 0, st, c->R, (int64_t)c->Ip, S, I, c->lse, nullptr, nullptr,
-                           Tpart, nullptr, nullptr, nullptr, 0);
+                           Tpart, nullptr, nullptr, nullptr, 0, nullptr, nullptr, 1, 1);
     hipLaunchKernelGGL(k_T_final, dim3((c->Sp + 255) / 256), dim3(256), 0, st, Tpart, ncb, S, c->Sp, c->T);
     if (c->mfma) hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, st, nzpart, (int)(ga.x * ga.y), c->scal + SC_NNZ);
     return hipGetLastError();

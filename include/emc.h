@@ -105,7 +105,15 @@ typedef struct emc_options {
                                 global->LDS loads (BF16 counts), 1 = staged through registers (byte
                                 counts widened in registers).  Same products in the same order:
                                 identical results */
-    int32_t reserved[1];
+    int32_t keep_logits;     /* emc_iterate with the MFMA likelihood: 0 = the logL kernel leaves out
+                                the stores of a 256-rotation band's logits for a pattern when the
+                                band's largest logit lies more than 128 below a largest logit
+                                already found for that pattern (its responsibilities there are
+                                exactly 0; the softmax reads those logits as -inf).  The iteration's
+                                results are the same; the logits are then incomplete and
+                                emc_get_logl after emc_iterate fails.  1 = every logit is stored.
+                                emc_logl always stores every logit, and the learned proposal's
+                                recall (which reads them) keeps them too */
 } emc_options;
 
 /* Slots of emc_iter_stats.stage_ms.  LOGL_GEMM / MSTEP_GEMM are the contraction kernels alone
@@ -143,6 +151,8 @@ typedef struct emc_iter_stats {
                               responsibility mass inside the cells the network would keep; else -1 */
     double train_loss;     /* mean cross-entropy of the last training pass; -1 if none ran */
     double work_fraction;  /* scored (rotation, pattern) pairs / (S * I) */
+    double logits_skipped; /* fraction of the (256-rotation band, pattern) logit blocks whose stores
+                              the logL kernel left out (options.keep_logits = 0); 0 otherwise */
 } emc_iter_stats;
 
 typedef struct emc_ctx emc_ctx;

@@ -616,6 +616,40 @@ def test_repeated_runs_are_bit_identical(small, prec):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("chunk,skips", [(512, True), (1000, True), (700, False)])
+def test_skipped_logit_stores_leave_the_iteration_unchanged(small, chunk, skips):
+    """emc_iterate leaves out the stores of a 256-rotation band's logits for a pattern whose band
+    maximum lies 128 nats below a maximum already found for it (keep_logits = 0, the default) and
+    packs the kept ones of a tile into whole-line blocks; the softmax reads the blocks and takes the
+    left-out bands as -inf.  The running maxima carry over the rotation chunks of a pass (at 3,240
+    rotations one launch runs every tile at once, so chunks are what exercise the skipping here);
+    chunks of a size that is not a multiple of 8 store every logit.  Over four iterations (the
+    posteriors sharpen), models, weights, log-sum-exps, responsibilities, slice sums and total
+    log-likelihoods equal those of keep_logits = 1 bit for bit, and the incomplete logits are not
+    handed out."""
+    log(f"[logit store skipping] chunk={chunk}")
+    d = small
+    outs, skipped = [], []
+    for keep in (1, 0):
+        with engine(d, "bf16x2", insert="binned", blur_sigma=0.5, slice_chunk=chunk, keep_logits=keep) as e:
+            sts = [e.iterate(it) for it in range(4)]
+            M, Wt = e.get_model()
+            outs.append(([s["total_logl"] for s in sts], M, Wt, e.get_lse(), e.get_resp(), e.get_slice_weights()))
+            skipped.append([s["logits_skipped"] for s in sts])
+            if keep or not skips:
+                e.get_logl()
+            else:
+                with pytest.raises(emc_amd.EmcError):
+                    e.get_logl()
+    log(f"   fraction of (band, pattern) logit blocks left out per iteration: {skipped[1]}")
+    assert skipped[0] == [0.0] * 4
+    assert (max(skipped[1]) > 0.1) if skips else skipped[1] == [0.0] * 4
+    a, b = outs
+    assert a[0] == b[0]
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
+
+
 def test_compress_of_given_sums_matches_oracle(orc):
     """emc_set_mstep_sums + emc_compress: divide, normalise over the voxels with weight, blur, on
     M-step sums with empty (zero-weight) voxels and zero numerators, against the oracle."""

@@ -179,7 +179,8 @@ def test_full_size_128_logits_and_iteration_properties(orc, c3):
     L, S, I = d["L"], d["q"].shape[0], d["K"].shape[0]
     log(f"[128^3 full size] {S} rotations x {I} patterns, max count {d['K'].max()}")
     n_pat = 16
-    with emc_amd.Engine(L, S, I, precision="bf16x2", blur_sigma=0.5) as e:
+    # keep_logits: every logit of the iteration is stored (read below)
+    with emc_amd.Engine(L, S, I, precision="bf16x2", blur_sigma=0.5, keep_logits=1) as e:
         e.setup(d["q"], d["w"], d["coords"], d["mask"], d["K"], d["m0"])
         st = e.iterate(0)
         R = e.get_logl_block(0, S, 0, n_pat)      # logits of the iteration's E-step (model m0)
