@@ -68,8 +68,11 @@ def run_engine(d, prec, steps, warmup, insert="binned", mfma_tile=0, **extra):
     e.close()
     ms = (t1 - t0) * 1000.0 / steps
     stage = {k: float(np.mean([s["stage_ms"][k] for s in stats])) for k in stats[0]["stage_ms"]}
-    log(f"[bench] {tag}: {ms:.2f} ms/iter, stages {json.dumps({k: round(v, 3) for k, v in stage.items()})}")
-    return ms, stage, stats[-1]
+    skipped = [round(s["logits_skipped"], 4) for s in stats]
+    log(f"[bench] {tag}: {ms:.2f} ms/iter, stages {json.dumps({k: round(v, 3) for k, v in stage.items()})}, "
+        f"logit blocks left out {skipped}")
+    last = dict(stats[-1], logits_skipped_mean=float(np.mean(skipped)))
+    return ms, stage, last
 
 
 def stage_rooflines(st, S, I, Pa, L, ms_iter):
@@ -386,6 +389,8 @@ def main():
     ap.add_argument("--insert", default="binned", choices=["binned", "fused", "atomic"])
     ap.add_argument("--mfma-staging", type=int, default=0, choices=[0, 1],
                     help="logL contraction: 0 = DMA-staged operands (default), 1 = register-staged")
+    ap.add_argument("--keep-logits", type=int, default=0, choices=[0, 1],
+                    help="1 = store every logit (0: negligible bands' logit stores are left out)")
     ap.add_argument("--mstep", default="auto", choices=["auto", "dense", "sparse"],
                     help="M-step form of the headline run (auto: chosen on the device per iteration)")
     ap.add_argument("--variants", action="store_true", help="also time other tilings / insertion modes")
@@ -425,7 +430,7 @@ def main():
         f"(generated in {time.perf_counter() - t:.1f}s)")
 
     ms_mfma, st_mfma, last = run_engine(d, "bf16x2", args.steps, args.warmup, args.insert, mstep=args.mstep,
-                                        mfma_staging=args.mfma_staging)
+                                        mfma_staging=args.mfma_staging, keep_logits=args.keep_logits)
     fp32 = None
     if not args.no_fp32:
         ms32, st32, _ = run_engine(d, "fp32", args.fp32_steps, 1, "atomic")
@@ -489,6 +494,7 @@ def main():
     config = {"workload": args.workload, "model_side": L, "rotations": S, "patterns": I, "active_pixels": Pa,
               "photons": d["cfg"]["photons"], "precision": "bf16x2", "insert": args.insert, "mstep": args.mstep,
               "interp": "trilinear", "likelihood": "true_poisson", "blur_sigma": d["cfg"]["blur"],
+              "keep_logits": args.keep_logits, "logit_blocks_left_out": round(last["logits_skipped_mean"], 4),
               "parallelism": "single_gpu"}
     detail = {"stage_ms": st_mfma, "stage_rooflines": stages, "fp32": fp32, "variants": variants or None,
               "accuracy": acc, "proposal": prop, "roofline": roof, "cpu_baseline": cpu, "config": config,

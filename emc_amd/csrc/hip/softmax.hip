@@ -145,8 +145,9 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                                                   uint8_t *__restrict__ pdirty, int nseg, const float *__restrict__ pm,
                                                   const int16_t *__restrict__ blk, int chunk, int bpc)
 {
+    static_assert(kApplyRows == 8, "8 rows: two per wave in the row sums, 8 floats per block read");
     int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
-    __shared__ double red[4][kApplyRows];
+    __shared__ double rs[kApplyRows][256];  // per-thread row sums
     const int c0 = blockIdx.x * kApplyCols + threadIdx.x * 4;
     const int r0 = blockIdx.y * kApplyRows;
     const int seg = (blockIdx.x * kApplyCols) / 256 + (threadIdx.x >> 6);  // this wave's 256 patterns
@@ -268,22 +269,25 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         }
         rowsum[rr] = acc;
     }
+    // row sums over the workgroup's 1,024 patterns through LDS: wave w adds rows 2w and 2w + 1 (one
+    // per 32-lane half, eight values a lane, then five shuffle steps), in a fixed order
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int rr = 0; rr < kApplyRows; ++rr) {
-        const double t = wave_sum(rowsum[rr]);
-        if (lane == 0) red[wave][rr] = t;
-    }
+    for (int rr = 0; rr < kApplyRows; ++rr) rs[rr][threadIdx.x] = rowsum[rr];
     __shared__ int nzw[4];
     if (OUT == 1) {
         nz = wave_sum(nz);
         if (lane == 0) nzw[wave] = nz;
     }
     __syncthreads();
-    if (threadIdx.x < kApplyRows) {
-        const int s = r0 + threadIdx.x;
-        const int k = threadIdx.x;
-        if (s < S) Tpart[(size_t)blockIdx.x * S + s] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+    {
+        const int row = 2 * wave + (lane >> 5), part = lane & 31;
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) t += rs[row][part + 32 * q];
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+        if (part == 0 && r0 + row < S) Tpart[(size_t)blockIdx.x * S + r0 + row] = t;
     }
     // per-workgroup non-zero count, summed by one deterministic pass afterwards
     if (OUT == 1 && threadIdx.x == 0)
