@@ -146,12 +146,14 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                                                   const int16_t *__restrict__ blk, int chunk, int bpc)
 {
     static_assert(kApplyRows == 8, "8 rows: two per wave in the row sums, 8 floats per block read");
+    constexpr float kLog2e = 1.4426950408889634f;
     int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
     __shared__ double rs[kApplyRows][256];  // per-thread row sums
     const int c0 = blockIdx.x * kApplyCols + threadIdx.x * 4;
     const int r0 = blockIdx.y * kApplyRows;
-    const int seg = (blockIdx.x * kApplyCols) / 256 + (threadIdx.x >> 6);  // this wave's 256 patterns
-    const bool seg_live = seg * 256 < I;                                    // uniform over the wave
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int seg = (blockIdx.x * kApplyCols) / 256 + wave;  // this wave's 256 patterns
+    const bool seg_live = seg * 256 < I;                      // uniform over the wave
     // lse as a float pair: x - lh is exact wherever p is not negligible (x and lse within a factor
     // of 2, Sterbenz), so exp((x - lh) - ll) carries the double-precision argument to float rounding
     // (a column past I gets lh = +inf: its p is exp2(-inf) = 0 without a select per element)
@@ -165,11 +167,13 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         neg[k] = false;
     }
     int code[4] = {-2, -2, -2, -2};
-    int brow0 = 0;
+    int brow0 = 0, b0 = 0;
+    bool one_band = false;
     if (OUT == 1 && pm) {
         const int rl = min(r0 + kApplyRows - 1, S - 1);
-        const int b0 = (r0 / chunk) * bpc + (r0 % chunk) / 256, b1 = (rl / chunk) * bpc + (rl % chunk) / 256;
-        if (b0 == b1) {  // the workgroup's rows lie in one band (always, when blk is given)
+        b0 = (r0 / chunk) * bpc + (r0 % chunk) / 256;
+        one_band = b0 == (rl / chunk) * bpc + (rl % chunk) / 256;
+        if (one_band) {  // the workgroup's rows lie in one band (always, when blk is given)
             brow0 = (r0 / chunk) * chunk + ((r0 % chunk) / 256) * 256;
 #pragma unroll
             for (int k = 0; k < 4; ++k)
@@ -180,100 +184,169 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                 }
         }
     }
-    // every row's logits and segment flag are loaded before the first is used, so the wave waits for
-    // memory once rather than once per row
-    // (rows past S read row S - 1 and a segment past the last reads the last; their values are not
-    // used)
-    float xv[kApplyRows][4];
+    // segment flags (a segment past the last reads the last; rows past S read row S - 1: not used)
     int flag[kApplyRows];
     const int segc = OUT == 1 ? min(seg, nseg - 1) : 0;
-    const bool in_place = code[0] == -2 && code[1] == -2 && code[2] == -2 && code[3] == -2;
-    const bool read = !(neg[0] && neg[1] && neg[2] && neg[3]);
 #pragma unroll
-    for (int rr = 0; rr < kApplyRows; ++rr) {
-        const size_t sc = (size_t)min(r0 + rr, S - 1);
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (c0 < I && read && in_place) v = *reinterpret_cast<const float4 *>(R + sc * ldr + c0);
-        xv[rr][0] = v.x;
-        xv[rr][1] = v.y;
-        xv[rr][2] = v.z;
-        xv[rr][3] = v.w;
-        flag[rr] = OUT == 1 ? (int)pdirty[sc * nseg + segc] : 0;
-    }
-    if (!in_place)
+    for (int rr = 0; rr < kApplyRows; ++rr) flag[rr] = OUT == 1 ? (int)pdirty[(size_t)min(r0 + rr, S - 1) * nseg + segc] : 0;
+
+    // Sparse form: when at most kSparseMax of the workgroup's patterns have a band that is not
+    // negligible, one thread per such pattern computes its 8 values (the others are exactly 0), the
+    // values go to LDS, and the P row segments that have to be written are assembled from there.
+    constexpr int kSparseMax = 256;
+    __shared__ int16_t amap[kApplyCols];               // local pattern -> its slot, -1 if negligible
+    __shared__ int16_t alist[kSparseMax];              // slot -> local pattern
+    __shared__ uint32_t aval[kSparseMax][kApplyRows];  // hi | lo << 16 per slot and row
+    __shared__ int wcnt[4];
+    __shared__ uint32_t nzf[kApplyRows];               // bit g: segment g of the row holds a non-zero
+    bool sparse = false;
+    if (OUT == 1 && pm && !iphi && one_band) {  // uniform over the workgroup
+        bool act[4];
+        int below = 0, wc = 0;
+        const uint64_t lt = (1ull << lane) - 1ull;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (neg[k]) continue;
-            if (code[k] >= 0) {  // 8 consecutive floats of the pattern's block
-                const float *b = R + (size_t)(brow0 + code[k]) * ldr + ((c0 + k) & ~255) + (r0 - brow0);
-                const float4 a0 = *reinterpret_cast<const float4 *>(b), a1 = *reinterpret_cast<const float4 *>(b + 4);
-                xv[0][k] = a0.x; xv[1][k] = a0.y; xv[2][k] = a0.z; xv[3][k] = a0.w;
-                xv[4][k] = a1.x; xv[5][k] = a1.y; xv[6][k] = a1.z; xv[7][k] = a1.w;
-            } else {
-#pragma unroll
-                for (int rr = 0; rr < kApplyRows; ++rr) xv[rr][k] = R[(size_t)min(r0 + rr, S - 1) * ldr + c0 + k];
-            }
+            act[k] = c0 + k < I && !neg[k];
+            const uint64_t bal = __ballot(act[k]);
+            wc += __popcll(bal);
+            below += __popcll(bal & lt);
         }
-    double rowsum[kApplyRows];
+        if (lane == 0) wcnt[wave] = wc;
+        if (threadIdx.x < kApplyRows) nzf[threadIdx.x] = 0u;
+        __syncthreads();
+        const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        sparse = total <= kSparseMax;
+        if (sparse) {
+            int slot = below;
+            for (int w = 0; w < wave; ++w) slot += wcnt[w];
 #pragma unroll
-    for (int rr = 0; rr < kApplyRows; ++rr) {
-        const int s = r0 + rr;
-        double acc = 0.0;
-        uint2 hv = make_uint2(0u, 0u), lv = make_uint2(0u, 0u);
-        int any_nz = 0;
-        if (s < S && c0 < I) {
-            float *src = R + (size_t)s * ldr + c0;
-            const float x[4] = {neg[0] ? -INFINITY : xv[rr][0], neg[1] ? -INFINITY : xv[rr][1],
-                                neg[2] ? -INFINITY : xv[rr][2], neg[3] ? -INFINITY : xv[rr][3]};
-            // exp(d) = 2^(d log2 e) on the hardware exp2: relative error about |d| 6e-8, below the
-            // 2^-16 of the BF16 pair P is stored as
-            constexpr float kLog2e = 1.4426950408889634f;
-            float p[4];
+            for (int k = 0; k < 4; ++k) {
+                amap[4 * threadIdx.x + k] = (int16_t)(act[k] ? slot : -1);
+                if (act[k]) alist[slot++] = (int16_t)(4 * threadIdx.x + k);
+            }
+            __syncthreads();
+            double ts[kApplyRows];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) p[k] = __builtin_amdgcn_exp2f(((x[k] - lh[k]) - ll[k]) * kLog2e);
-            acc = (double)((p[0] + p[1]) + (p[2] + p[3]));
-            if (OUT == 0) {
-                *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
-            } else {
-                const size_t o = (size_t)s * ldr + c0;
-                float q[4] = {p[0], p[1], p[2], p[3]};
-                if (iphi) {
-                    const float4 f = *reinterpret_cast<const float4 *>(iphi + o);
-                    q[0] *= f.x; q[1] *= f.y; q[2] *= f.z; q[3] *= f.w;
+            for (int rr = 0; rr < kApplyRows; ++rr) ts[rr] = 0.0;
+            if (threadIdx.x < total) {
+                const int lp = alist[threadIdx.x], n = blockIdx.x * kApplyCols + lp;
+                const double l = lse[n];
+                const float lhn = (float)l, lln = (float)(l - (double)lhn);
+                const int cd = blk ? (int)blk[(size_t)b0 * I + n] : -2;
+                float x[kApplyRows];
+                if (cd >= 0) {
+                    const float *b = R + (size_t)(brow0 + cd) * ldr + (n & ~255) + (r0 - brow0);
+                    const float4 a0 = *reinterpret_cast<const float4 *>(b), a1 = *reinterpret_cast<const float4 *>(b + 4);
+                    x[0] = a0.x; x[1] = a0.y; x[2] = a0.z; x[3] = a0.w;
+                    x[4] = a1.x; x[5] = a1.y; x[6] = a1.z; x[7] = a1.w;
+                } else {
+#pragma unroll
+                    for (int rr = 0; rr < kApplyRows; ++rr) x[rr] = R[(size_t)min(r0 + rr, S - 1) * ldr + n];
                 }
-                // hi of a pair in one v_cvt_pk_bf16_f32; its two values back as floats by a shift and a
-                // mask; lo = bf16(q - hi).  q >= 0, so (hi | lo) != 0 exactly when hi != 0.
-                const uint32_t h01 = pk_bf16(q[0], q[1]), h23 = pk_bf16(q[2], q[3]);
-                const uint32_t l01 = pk_bf16(q[0] - __builtin_bit_cast(float, h01 << 16),
-                                             q[1] - __builtin_bit_cast(float, h01 & 0xffff0000u));
-                const uint32_t l23 = pk_bf16(q[2] - __builtin_bit_cast(float, h23 << 16),
-                                             q[3] - __builtin_bit_cast(float, h23 & 0xffff0000u));
-                const int mine = ((h01 & 0xffffu) != 0) + ((h01 >> 16) != 0) + ((h23 & 0xffffu) != 0) +
-                                 ((h23 >> 16) != 0);
-                nz += mine;
-                hv = make_uint2(h01, h23);
-                lv = make_uint2(l01, l23);
-                any_nz = mine != 0;
+#pragma unroll
+                for (int rr = 0; rr < kApplyRows; ++rr) {
+                    const float p = r0 + rr < S ? __builtin_amdgcn_exp2f(((x[rr] - lhn) - lln) * kLog2e) : 0.0f;
+                    ts[rr] = (double)p;
+                    // the dense form's rounding: hi = bf16(p), lo = bf16(p - hi), one conversion each
+                    const uint32_t h = pk_bf16(p, 0.0f) & 0xffffu;
+                    const uint32_t lo = pk_bf16(p - __builtin_bit_cast(float, h << 16), 0.0f) & 0xffffu;
+                    aval[threadIdx.x][rr] = h | (lo << 16);
+                    if (h != 0u) {
+                        ++nz;
+                        atomicOr(&nzf[rr], 1u << (lp >> 8));
+                    }
+                }
             }
+#pragma unroll
+            for (int rr = 0; rr < kApplyRows; ++rr) rs[rr][threadIdx.x] = ts[rr];
         }
-        if (OUT == 1 && s < S && seg_live) {  // s and seg_live are uniform over the wave
-            const bool seg_nz = __any(any_nz) != 0;
-            const size_t fo = (size_t)s * nseg + seg;
-            const bool dirty = flag[rr] != 0;
-            if ((seg_nz || dirty) && c0 < I) {
-                const size_t o = (size_t)s * ldr + c0;
-                *reinterpret_cast<uint2 *>(Ph + o) = hv;
-                *reinterpret_cast<uint2 *>(Pl + o) = lv;
+    }
+    if (!sparse) {
+        // every row's logits are loaded before the first is used, so the wave waits for memory once
+        // rather than once per row (rows past S read row S - 1: not used)
+        float xv[kApplyRows][4];
+        const bool in_place = code[0] == -2 && code[1] == -2 && code[2] == -2 && code[3] == -2;
+        const bool read = !(neg[0] && neg[1] && neg[2] && neg[3]);
+#pragma unroll
+        for (int rr = 0; rr < kApplyRows; ++rr) {
+            const size_t sc = (size_t)min(r0 + rr, S - 1);
+            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (c0 < I && read && in_place) v = *reinterpret_cast<const float4 *>(R + sc * ldr + c0);
+            xv[rr][0] = v.x;
+            xv[rr][1] = v.y;
+            xv[rr][2] = v.z;
+            xv[rr][3] = v.w;
+        }
+        if (!in_place)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (neg[k]) continue;
+                if (code[k] >= 0) {  // 8 consecutive floats of the pattern's block
+                    const float *b = R + (size_t)(brow0 + code[k]) * ldr + ((c0 + k) & ~255) + (r0 - brow0);
+                    const float4 a0 = *reinterpret_cast<const float4 *>(b), a1 = *reinterpret_cast<const float4 *>(b + 4);
+                    xv[0][k] = a0.x; xv[1][k] = a0.y; xv[2][k] = a0.z; xv[3][k] = a0.w;
+                    xv[4][k] = a1.x; xv[5][k] = a1.y; xv[6][k] = a1.z; xv[7][k] = a1.w;
+                } else {
+#pragma unroll
+                    for (int rr = 0; rr < kApplyRows; ++rr) xv[rr][k] = R[(size_t)min(r0 + rr, S - 1) * ldr + c0 + k];
+                }
             }
-            if ((threadIdx.x & 63) == 0 && dirty != seg_nz) pdirty[fo] = seg_nz ? 1 : 0;
+#pragma unroll
+        for (int rr = 0; rr < kApplyRows; ++rr) {
+            const int s = r0 + rr;
+            double acc = 0.0;
+            uint2 hv = make_uint2(0u, 0u), lv = make_uint2(0u, 0u);
+            int any_nz = 0;
+            if (s < S && c0 < I) {
+                float *src = R + (size_t)s * ldr + c0;
+                const float x[4] = {neg[0] ? -INFINITY : xv[rr][0], neg[1] ? -INFINITY : xv[rr][1],
+                                    neg[2] ? -INFINITY : xv[rr][2], neg[3] ? -INFINITY : xv[rr][3]};
+                // exp(d) = 2^(d log2 e) on the hardware exp2: relative error about |d| 6e-8, below the
+                // 2^-16 of the BF16 pair P is stored as
+                float p[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) p[k] = __builtin_amdgcn_exp2f(((x[k] - lh[k]) - ll[k]) * kLog2e);
+                acc = (double)((p[0] + p[1]) + (p[2] + p[3]));
+                if (OUT == 0) {
+                    *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
+                } else {
+                    const size_t o = (size_t)s * ldr + c0;
+                    float q[4] = {p[0], p[1], p[2], p[3]};
+                    if (iphi) {
+                        const float4 f = *reinterpret_cast<const float4 *>(iphi + o);
+                        q[0] *= f.x; q[1] *= f.y; q[2] *= f.z; q[3] *= f.w;
+                    }
+                    // hi of a pair in one v_cvt_pk_bf16_f32; its two values back as floats by a shift
+                    // and a mask; lo = bf16(q - hi).  q >= 0, so (hi | lo) != 0 exactly when hi != 0.
+                    const uint32_t h01 = pk_bf16(q[0], q[1]), h23 = pk_bf16(q[2], q[3]);
+                    const uint32_t l01 = pk_bf16(q[0] - __builtin_bit_cast(float, h01 << 16),
+                                                 q[1] - __builtin_bit_cast(float, h01 & 0xffff0000u));
+                    const uint32_t l23 = pk_bf16(q[2] - __builtin_bit_cast(float, h23 << 16),
+                                                 q[3] - __builtin_bit_cast(float, h23 & 0xffff0000u));
+                    const int mine = ((h01 & 0xffffu) != 0) + ((h01 >> 16) != 0) + ((h23 & 0xffffu) != 0) +
+                                     ((h23 >> 16) != 0);
+                    nz += mine;
+                    hv = make_uint2(h01, h23);
+                    lv = make_uint2(l01, l23);
+                    any_nz = mine != 0;
+                }
+            }
+            if (OUT == 1 && s < S && seg_live) {  // s and seg_live are uniform over the wave
+                const bool seg_nz = __any(any_nz) != 0;
+                const size_t fo = (size_t)s * nseg + seg;
+                const bool dirty = flag[rr] != 0;
+                if ((seg_nz || dirty) && c0 < I) {
+                    const size_t o = (size_t)s * ldr + c0;
+                    *reinterpret_cast<uint2 *>(Ph + o) = hv;
+                    *reinterpret_cast<uint2 *>(Pl + o) = lv;
+                }
+                if (lane == 0 && dirty != seg_nz) pdirty[fo] = seg_nz ? 1 : 0;
+            }
+            rs[rr][threadIdx.x] = acc;
         }
-        rowsum[rr] = acc;
     }
     // row sums over the workgroup's 1,024 patterns through LDS: wave w adds rows 2w and 2w + 1 (one
     // per 32-lane half, eight values a lane, then five shuffle steps), in a fixed order
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-    for (int rr = 0; rr < kApplyRows; ++rr) rs[rr][threadIdx.x] = rowsum[rr];
     __shared__ int nzw[4];
     if (OUT == 1) {
         nz = wave_sum(nz);
@@ -288,6 +361,29 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
 #pragma unroll
         for (int o = 16; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
         if (part == 0 && r0 + row < S) Tpart[(size_t)blockIdx.x * S + r0 + row] = t;
+    }
+    if (sparse) {  // the P row segments that hold a non-zero or held one, assembled from the slots
+#pragma unroll
+        for (int rr = 0; rr < kApplyRows; ++rr) {
+            const int s = r0 + rr;
+            if (s >= S || !seg_live) continue;
+            const bool seg_nz = (nzf[rr] >> wave) & 1u;
+            const bool dirty = flag[rr] != 0;
+            if ((seg_nz || dirty) && c0 < I) {
+                uint32_t v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int a = amap[4 * threadIdx.x + k];
+                    v[k] = a >= 0 ? aval[a][rr] : 0u;
+                }
+                const size_t o = (size_t)s * ldr + c0;
+                *reinterpret_cast<uint2 *>(Ph + o) =
+                    make_uint2((v[0] & 0xffffu) | (v[1] << 16), (v[2] & 0xffffu) | (v[3] << 16));
+                *reinterpret_cast<uint2 *>(Pl + o) =
+                    make_uint2((v[0] >> 16) | (v[1] & 0xffff0000u), (v[2] >> 16) | (v[3] & 0xffff0000u));
+            }
+            if (lane == 0 && dirty != seg_nz) pdirty[(size_t)s * nseg + seg] = seg_nz ? 1 : 0;
+        }
     }
     // per-workgroup non-zero count, summed by one deterministic pass afterwards
     if (OUT == 1 && threadIdx.x == 0)

@@ -267,6 +267,43 @@ def test_full_iterations_match_oracle(orc, small, prec):
             m = ref["model"].astype(np.float32)
 
 
+def test_sharp_posteriors_match_oracle(orc, small):
+    """Once the model has formed, most (256-rotation band, pattern) blocks are negligible: the logL
+    pass leaves their stores out and the softmax takes the workgroups with few patterns that are not
+    negligible in its sparse form (one thread per such pattern, P staged in LDS).  An iteration from the
+    model of the sixth, and the stage-wise responsibilities from that model (every logit stored, the
+    same sparse softmax), against the oracle from the same model."""
+    log("[sharp posteriors]")
+    d = small
+    L = d["L"]
+    S, I = d["q"].shape[0], d["K"].shape[0]
+    K = d["K"].astype(np.float64)
+    w64 = d["w"].astype(np.float64)
+    with engine(d, "bf16x2", blur_sigma=0.5, slice_chunk=1000) as e:
+        for it in range(6):
+            e.iterate(it)
+        m = e.get_model()[0].copy()
+        st = e.iterate(6)
+        M, _ = e.get_model()
+        P, T = e.get_resp(), e.get_slice_weights()
+        e.set_model(m)
+        e.logl()
+        tot = e.normalize_resp()
+        P2, lse2, T2 = e.get_resp(), e.get_lse(), e.get_slice_weights()
+    ref = orc.iterate(m.astype(np.float64), L, d["q"], w64, K, d["mask"], d["coords"], "trilinear",
+                      "true_poisson", blur_sigma=0.5, want_P=True)
+    mt = metrics(None, None, P, ref["P"], M, ref["model"])
+    l1 = np.abs(P2 - ref["P"]).sum(0)
+    log(f"   iteration 6: {mt}, logit blocks left out {st['logits_skipped']:.3f}; stages: resp L1 max "
+        f"{l1.max():.2e}, total logL rel {abs(tot - ref['total']) / abs(ref['total']):.2e}")
+    assert st["logits_skipped"] > 0.3
+    assert mt["resp_l1_max"] < 5e-3 and mt["model_rel_l2"] < 1e-4
+    assert abs(st["total_logl"] - ref["total"]) < 1e-5 * abs(ref["total"])
+    assert l1.max() < 5e-3 and abs(tot - ref["total"]) < 1e-5 * abs(ref["total"])
+    assert np.array_equal(P, P2) and np.array_equal(T, T2)  # same model, same E-step arithmetic
+    assert np.allclose(T, P.astype(np.float64).sum(1), rtol=1e-3, atol=1e-6)
+
+
 def test_scaled_iterations_match_oracle(orc, small):
     """Two iterations with known_intensity = 0 and the reference's chi^2 ("poisson") diff: the
     scaling enters the logits and the M-step sums K P / phi (emc_atomic.cu:107-108)."""

@@ -164,6 +164,37 @@ def test_full_size_64_iteration_on_2000_patterns(orc, c2, prec):
     assert mt["weight_rel_l2"] < 1e-4
 
 
+@pytest.mark.timeout(600)
+def test_full_size_64_formed_model_iteration_on_2000_patterns(orc, c2):
+    """After six iterations the posteriors are sharp: the iteration leaves out most logit blocks and
+    the softmax takes its sparse form in most workgroups.  The seventh iteration against the oracle's
+    from the same model (responsibilities over every rotation, model, total logL)."""
+    d = c2
+    S, I = d["q"].shape[0], 2000
+    log(f"[64^3 formed model] bf16x2: {S} rotations x {I} patterns, iteration 6 against the oracle")
+    K = d["K"][:I]
+    with emc_amd.Engine(d["L"], S, I, precision="bf16x2", blur_sigma=0.5) as e:
+        e.setup(d["q"], d["w"], d["coords"], d["mask"], K, d["m0"])
+        for it in range(6):
+            e.iterate(it)
+        m = e.get_model()[0].copy()
+        st = e.iterate(6)
+        M, Wt = e.get_model()
+        P = e.get_resp()
+    ref = orc.iterate(m.astype(np.float64), d["L"], d["q"], d["w"].astype(np.float64), K.astype(np.float64),
+                      d["mask"], d["coords"], "trilinear", "true_poisson", blur_sigma=0.5, want_P=True)
+    l1 = np.abs(P.astype(np.float64) - ref["P"]).sum(0)
+    mt = {"model_rel_l2": rel_l2(M, ref["model"]), "weight_rel_l2": rel_l2(Wt, ref["Wt"]),
+          "total_logl_rel": abs(st["total_logl"] - ref["total"]) / abs(ref["total"]),
+          "resp_l1_max": float(l1.max()), "logit_blocks_left_out": st["logits_skipped"]}
+    log("   " + ", ".join(f"{k} {v:.2e}" for k, v in mt.items()))
+    assert mt["logit_blocks_left_out"] > 0.5
+    assert mt["total_logl_rel"] < 1e-5
+    assert mt["resp_l1_max"] < 5e-3
+    assert mt["model_rel_l2"] < 1e-4
+    assert mt["weight_rel_l2"] < 1e-4
+
+
 # ---- 128^3 / 50,100 / 50,000, 300 photons ------------------------------------------------------
 
 @pytest.fixture(scope="module")
