@@ -196,7 +196,7 @@ extern "C" void emc_destroy(emc_ctx *c)
     dfree(c->q); dfree(c->rotm); dfree(c->logw); dfree(c->coords); dfree(c->mask); dfree(c->axyz); dfree(c->pix);
     dfree(c->K32); dfree(c->KT32); dfree(c->K16); dfree(c->K16lo); dfree(c->KT16); dfree(c->KT16lo); dfree(c->sumK);
     dfree(c->K8); dfree(c->KT8);
-    dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->taps);
+    dfree(c->model); dfree(c->Mnum); dfree(c->Wt); dfree(c->tmp3); dfree(c->modelq); dfree(c->taps);
     dfree(c->sumW); dfree(c->ns); dfree(c->colsum); dfree(c->colp); dfree(c->colpart); dfree(c->colmean); dfree(c->bias); dfree(c->rscale); dfree(c->cst);
     dfree(c->A0); dfree(c->A1); dfree(c->Am); dfree(c->AWm); dfree(c->K2); dfree(c->iphi); dfree(c->Pphi); dfree(c->Lh); dfree(c->Lm); dfree(c->Ll); dfree(c->lw32);
     dfree(c->Cx); dfree(c->K2h); dfree(c->K2l); dfree(c->Aw); dfree(c->G2); dfree(c->G3); dfree(c->G4);
@@ -322,6 +322,7 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (st == EMC_OK) A(dalloc(&c->Mnum, N3), "Mnum");
     if (st == EMC_OK) A(dalloc(&c->Wt, N3), "Wt");
     if (st == EMC_OK) A(dalloc(&c->tmp3, N3), "tmp3");
+    if (st == EMC_OK) A(dalloc(&c->modelq, N3), "modelq");
     if (st == EMC_OK) A(dalloc(&c->taps, c->L), "taps");
     if (st == EMC_OK) A(dalloc(&c->sumW, Sp), "sumW");
     if (st == EMC_OK) A(dalloc(&c->ns, Sp), "ns");

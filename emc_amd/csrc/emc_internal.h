@@ -151,6 +151,7 @@ struct emc_ctx {
     double *sumK = nullptr;   // [Ip]
     // model
     float *model = nullptr, *Mnum = nullptr, *Wt = nullptr, *tmp3 = nullptr;  // [N3]
+    float4 *modelq = nullptr; // [N3] quad-packed model of the trilinear statistics pass (MFMA path)
     float *taps = nullptr;    // [L] blur taps
     // slicing stage
     double *sumW = nullptr;   // [Sp]

@@ -84,6 +84,31 @@ __device__ __forceinline__ float sample(const float *__restrict__ model, int L, 
     }
 }
 
+// Trilinear sample from the quad-packed model (modelq[v] = model at v, v + 1, v + L, v + L + 1; 0 past
+// an edge): the 8 corners in two 16-byte loads from two cache lines instead of 8 loads from four.  The
+// same corners, test and arithmetic as sample<EMC_INTERP_TRILINEAR>, so the same value bit for bit.
+__device__ __forceinline__ float sample_quad(const float4 *__restrict__ modelq, int L, const float *m, float X,
+                                             float Y, float Z)
+{
+    float ux, uy, uz;
+    rot_point(m, X, Y, Z, model_c(L), ux, uy, uz);
+    int bx, by, bz;
+    float fx, fy, fz;
+    if (!tri_axis(ux, L, bx, fx) || !tri_axis(uy, L, by, fy) || !tri_axis(uz, L, bz, fz)) return -1.0f;
+    const size_t base = ((size_t)bz * L + by) * L + bx;
+    const float4 q0 = modelq[base], q1 = modelq[base + (size_t)L * L];
+    const float v000 = q0.x, v001 = q0.y, v010 = q0.z, v011 = q0.w;
+    const float v100 = q1.x, v101 = q1.y, v110 = q1.z, v111 = q1.w;
+    const float mn = fminf(fminf(fminf(v000, v001), fminf(v010, v011)),
+                           fminf(fminf(v100, v101), fminf(v110, v111)));
+    if (mn < 0.0f) return -1.0f;  // a missing voxel among the corners
+    const float gx = 1.0f - fx, gy = 1.0f - fy, gz = 1.0f - fz;
+    const float c00 = v000 * gx + v001 * fx, c01 = v010 * gx + v011 * fx;
+    const float c10 = v100 * gx + v101 * fx, c11 = v110 * gx + v111 * fx;
+    const float c0 = c00 * gy + c01 * fy, c1 = c10 * gy + c11 * fy;
+    return c0 * gz + c1 * fz;
+}
+
 __device__ __forceinline__ void load_rot(const float *__restrict__ rotm, int s, float *m)
 {
     const float4 a = *reinterpret_cast<const float4 *>(rotm + 12 * (size_t)s);

@@ -54,8 +54,21 @@ __global__ void k_fill(T *__restrict__ p, T v, int64_t n)
 // kStatsG rotations per workgroup; each thread walks pixels j = tid, tid+256, ... and keeps the
 // kStatsG partial sums in registers.  With lw, the pass also keeps log W ([Sp][Pp], -inf where
 // W <= 0) for the operand split, which then needs no second gather from the model.
+// modelq[v] = (model[v], model[v + 1], model[v + L], model[v + L + 1]), 0 past an edge (the trilinear
+// statistics pass reads the 8 corners of a sample as two 16-byte loads).
+__global__ void k_quad_model(const float *__restrict__ model, int L, float4 *__restrict__ modelq)
+{
+    const int64_t n = (int64_t)L * L * L;
+    for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x) {
+        const bool xe = (int)(v % L) + 1 < L, ye = (int)((v / L) % L) + 1 < L;
+        modelq[v] = make_float4(model[v], xe ? model[v + 1] : 0.0f, ye ? model[v + L] : 0.0f,
+                                xe && ye ? model[v + L + 1] : 0.0f);
+    }
+}
+
 template <int INTERP>
-__global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ model, int L,
+__global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ model,
+                                                     const float4 *__restrict__ modelq, int L,
                                                      const float *__restrict__ rotm, int S,
                                                      const float *__restrict__ axyz, int Pa, int Pp,
                                                      double *__restrict__ sumW, int32_t *__restrict__ ns,
@@ -83,7 +96,8 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
 #pragma unroll
         for (int g = 0; g < kStatsG; ++g) {
             if (g < ng) {
-                const float W = sample<INTERP>(model, L, rm[g], X, Y, Z);
+                const float W = INTERP == EMC_INTERP_TRILINEAR ? sample_quad(modelq, L, rm[g], X, Y, Z)
+                                                               : sample<INTERP>(model, L, rm[g], X, Y, Z);
                 float l = -INFINITY;
                 if (W > 0.0f) {
                     l = logf(W);
@@ -439,12 +453,16 @@ hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st
 {
     const int nblk = (c->S + kStatsG - 1) / kStatsG;
     float2 *cp = want_colsum ? c->colp : nullptr;
-    if (c->opt.interp == EMC_INTERP_NEAREST)
-        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_NEAREST>, dim3(nblk), dim3(256), 0, st, c->model, c->L,
+    if (c->opt.interp == EMC_INTERP_NEAREST) {
+        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_NEAREST>, dim3(nblk), dim3(256), 0, st, c->model, nullptr, c->L,
                            c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cp, c->lw32);
-    else
-        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_TRILINEAR>, dim3(nblk), dim3(256), 0, st, c->model, c->L,
-                           c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cp, c->lw32);
+    } else {
+        const int64_t n3 = (int64_t)c->L * c->L * c->L;
+        hipLaunchKernelGGL(k_quad_model, dim3((unsigned)std::min<int64_t>((n3 + 255) / 256, 4096)), dim3(256), 0, st,
+                           c->model, c->L, c->modelq);
+        hipLaunchKernelGGL(k_slice_stats<EMC_INTERP_TRILINEAR>, dim3(nblk), dim3(256), 0, st, c->model, c->modelq,
+                           c->L, c->rotm, c->S, c->axyz, c->Pa, c->Pp, c->sumW, c->ns, cp, c->lw32);
+    }
     if (want_colsum) {
         hipLaunchKernelGGL(k_colsum_part, dim3((c->Pa + 255) / 256, kColParts), dim3(256), 0, st, c->colp, nblk, c->Pa,
                            c->Pp, c->colpart);
