@@ -15,6 +15,7 @@
 // banks; the next k-tile is fetched into registers while the current one is multiplied.
 // The fused-scatter epilogue stages the accumulators through LDS, 64 rows at a time.
 #include <algorithm>
+#include <type_traits>
 
 #include "../emc_internal.h"
 #include "geom.h"
@@ -637,17 +638,38 @@ __device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, con
     __shared__ float bias_s[BM2];
     __shared__ int32_t rm_s[BN2];
 
-    // tile order as in k_gemm_mfma2
+    // Tile order as in k_gemm_mfma2 (XCD x = b % 8 takes its contiguous share of tiles in the order
+    // of b / 8), except that a last row band holding padded rows is left to the end of every XCD's
+    // share (its ntn tiles split over the XCDs): the waves of its padded rows skip their products,
+    // so those tiles cost about half, and each XCD's final, partly filled round is made of them.
     const int nwg = gridDim.x, b = blockIdx.x;
     const int base = nwg / 8, rem = nwg % 8, xcd = b % 8, loc = b / 8;
-    const int t = xcd * base + min(xcd, rem) + loc;
-    const int ntm = nwg / ntn, gi = t / (gm * ntn), gr = t % (gm * ntn);
-    const int rows = min(gm, ntm - gi * gm);
-    const int m0 = (gi * gm + gr % rows) * BM2, n0 = (gr / rows) * BN2;
+    const int ntm = nwg / ntn;
+    int m0, n0;
+    if (e.m_valid < ntm * BM2 && ntm > 1 && ntn >= 8) {  // then every XCD has room for its share
+        const int pc = ntn / 8 + (xcd < ntn % 8 ? 1 : 0), pstart = xcd * (ntn / 8) + min(xcd, ntn % 8);
+        const int fc = base + (xcd < rem ? 1 : 0) - pc, fstart = xcd * base + min(xcd, rem) - pstart;
+        if (loc < fc) {
+            const int t = fstart + loc, gi = t / (gm * ntn), gr = t % (gm * ntn);
+            const int rows = min(gm, ntm - 1 - gi * gm);
+            m0 = (gi * gm + gr % rows) * BM2;
+            n0 = (gr / rows) * BN2;
+        } else {
+            m0 = (ntm - 1) * BM2;
+            n0 = (pstart + loc - fc) * BN2;
+        }
+    } else {
+        const int t = xcd * base + min(xcd, rem) + loc;
+        const int gi = t / (gm * ntn), gr = t % (gm * ntn);
+        const int rows = min(gm, ntm - gi * gm);
+        m0 = (gi * gm + gr % rows) * BM2;
+        n0 = (gr / rows) * BN2;
+    }
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * (BN2 / 2);
+    const bool live = m0 + wm < e.m_valid;  // a wave whose 64 rows are all padding multiplies nothing
     // DMA instruction q of this wave: operand op (A hi, A lo, B), rows 16 rb .. 16 rb + 15; lane l
     // fetches logical chunk (l & 3) ^ lds_swz(r) of row r = 16 rb + l / 4
     const bf16_t *src[LPW];
@@ -716,33 +738,42 @@ __device__ __forceinline__ void mfma_dma_body(const bf16_t *__restrict__ A0, con
     if (tid < BM2) bias_s[tid] = bv;
     if (tid < BN2) rm_s[tid] = rv;
     __syncthreads();
-    rd(F0, 0, 0);
-    int k = 0;
-    for (; k + 2 < nk; ++k) {
-        rd(F1, k % kDmaStages, 16);
-        issue(k + 2, (k + 2) % kDmaStages);
-        __builtin_amdgcn_sched_barrier(0);
-        mm(F0);
-        __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
-        __syncthreads();
-        rd(F0, (k + 1) % kDmaStages, 0);
-        __builtin_amdgcn_sched_barrier(0);
-        mm(F1);
-    }
-    if (k + 1 < nk) {
-        rd(F1, k % kDmaStages, 16);
-        __builtin_amdgcn_sched_barrier(0);
-        mm(F0);
-        __builtin_amdgcn_s_waitcnt(kVm0);
-        __syncthreads();
-        rd(F0, (k + 1) % kDmaStages, 0);
-        __builtin_amdgcn_sched_barrier(0);
-        mm(F1);
-        ++k;
-    }
-    rd(F1, k % kDmaStages, 16);
-    mm(F0);
-    mm(F1);
+    // the k loop; a wave of padded rows (LIVE false) only issues its share of the loads and keeps
+    // the barriers
+    auto kloop = [&](auto live_tag) {
+        constexpr bool LIVE = decltype(live_tag)::value;
+        if (LIVE) rd(F0, 0, 0);
+        int k = 0;
+        for (; k + 2 < nk; ++k) {
+            if (LIVE) rd(F1, k % kDmaStages, 16);
+            issue(k + 2, (k + 2) % kDmaStages);
+            __builtin_amdgcn_sched_barrier(0);
+            if (LIVE) mm(F0);
+            __builtin_amdgcn_s_waitcnt(kVm0 | LPW);
+            __syncthreads();
+            if (LIVE) rd(F0, (k + 1) % kDmaStages, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (LIVE) mm(F1);
+        }
+        if (k + 1 < nk) {
+            if (LIVE) rd(F1, k % kDmaStages, 16);
+            __builtin_amdgcn_sched_barrier(0);
+            if (LIVE) mm(F0);
+            __builtin_amdgcn_s_waitcnt(kVm0);
+            __syncthreads();
+            if (LIVE) rd(F0, (k + 1) % kDmaStages, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (LIVE) mm(F1);
+            ++k;
+        }
+        if (LIVE) {
+            rd(F1, k % kDmaStages, 16);
+            mm(F0);
+            mm(F1);
+        }
+    };
+    if (live) kloop(std::true_type{});
+    else kloop(std::false_type{});
     __syncthreads();  // the epilogue reuses the LDS
     if constexpr (EPI == EPI_LOGL) {
         logl_epilogue<TN, BN2>(acc, e, m0, n0, wm, wn, smem, bias_s, e.runmax ? rm_s : nullptr);
