@@ -75,8 +75,9 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
                                                      float2 *__restrict__ colp, float *__restrict__ lw)
 {
     __shared__ float rm[kStatsG][9];
-    __shared__ double red_w[4][kStatsG];
-    __shared__ int red_n[4][kStatsG];
+    __shared__ double red_w[kStatsG][256];
+    __shared__ int red_n[kStatsG][256];
+    static_assert(kStatsG * 32 == 256, "32 lanes per rotation in the final sums");
     const int s0 = blockIdx.x * kStatsG;
     const int tid = threadIdx.x;
     if (tid < kStatsG * 9) {
@@ -111,17 +112,33 @@ __global__ __launch_bounds__(256) void k_slice_stats(const float *__restrict__ m
         }
         if (colp) colp[(size_t)blockIdx.x * Pp + j] = make_float2(col, (float)colv);
     }
-    const int wave = tid >> 6, lane = tid & 63;
+    // per-rotation sums over the workgroup's 256 threads through LDS, in a fixed order: 32 lanes per
+    // rotation add 8 values each, then five shuffle steps
+    const int lane = tid & 63;
 #pragma unroll
     for (int g = 0; g < kStatsG; ++g) {
-        const double a = wave_sum((double)sw[g]);
-        const int b = wave_sum(cnt[g]);
-        if (lane == 0) { red_w[wave][g] = a; red_n[wave][g] = b; }
+        red_w[g][tid] = (double)sw[g];
+        red_n[g][tid] = cnt[g];
     }
     __syncthreads();
-    if (tid < ng) {
-        sumW[s0 + tid] = red_w[0][tid] + red_w[1][tid] + red_w[2][tid] + red_w[3][tid];
-        ns[s0 + tid] = red_n[0][tid] + red_n[1][tid] + red_n[2][tid] + red_n[3][tid];
+    {
+        const int g = tid >> 5, part = lane & 31;
+        double a = 0.0;
+        int b = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            a += red_w[g][part + 32 * q];
+            b += red_n[g][part + 32 * q];
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1) {
+            a += __shfl_xor(a, o, 64);
+            b += __shfl_xor(b, o, 64);
+        }
+        if (part == 0 && g < ng) {
+            sumW[s0 + g] = a;
+            ns[s0 + g] = b;
+        }
     }
 }
 
@@ -308,7 +325,9 @@ __global__ __launch_bounds__(256) void k_expand(const float *__restrict__ model,
         out[(size_t)r * P + p] = sample<INTERP>(model, L, m, coords[p], coords[P + p], coords[2 * P + p]);
 }
 
-// Fluence, per-rotation logit terms and pixel column means; one workgroup.
+// Fluence, per-rotation logit terms and pixel column means.  Every workgroup forms the total
+// sum_s sumW_s itself (the same fixed order in each, so the same value) and writes its share of the
+// rotations and pixels; workgroup 0 writes the scalars.
 //   TRUE_POISSON: phi0 = fluence or (sumK/I)/(sum_s sumW_s / S);
 //                 bias_s = log w_s - phi0 sumW_s + B0, B0 = phi0 * mean_s sumW_s (a global
 //                 shift that keeps the stored logits small; it is returned through cst).
@@ -343,14 +362,16 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
         else if (SW > 0.0 && I > 0 && sumK_total > 0.0) phi0 = (sumK_total / I) / (SW / S);  // else undefined: 1
     }
     const double B0 = (diff == EMC_DIFF_TRUE_POISSON && S > 0) ? phi0 * SW / S : 0.0;
-    if (tid == 0) {
+    if (blockIdx.x == 0 && tid == 0) {
         scal[SC_PHI0] = phi0;
         scal[SC_LOGPHI0] = log(phi0);
         scal[SC_SUMW] = SW;
+        scal[SC_B0] = B0;
     }
     const double inv2s2 = 1.0 / (2.0 * (double)sigma * (double)sigma);
+    const int t0 = blockIdx.x * 1024 + tid, stride = gridDim.x * 1024;
 #pragma unroll 4
-    for (int s = tid; s < Sp; s += 1024) {
+    for (int s = t0; s < Sp; s += stride) {
         float b = 0.0f, rs = 0.0f;
         if (s < S) {
             if (diff == EMC_DIFF_TRUE_POISSON) b = (float)((double)logw[s] - phi0 * sumW[s] + B0);
@@ -364,12 +385,11 @@ __global__ __launch_bounds__(1024) void k_bias(const double *__restrict__ sumW, 
     }
     // column mean of log(phi0 W) over rotations (invalid samples count as 0)
     const double lp = log(phi0);
-    for (int j = tid; j < Pp; j += 1024)
+    for (int j = t0; j < Pp; j += stride)
         colmean[j] = (colsum && j < Pa && S > 0) ? (float)((colsum[2 * j] + colsum[2 * j + 1] * lp) / S) : 0.0f;
-    if (tid == 0) scal[SC_B0] = B0;
 }
 
-// cst_i = sum_j K_ij colmean_j - B0   (FP64; one workgroup per pattern)
+// cst_i = sum_j K_ij colmean_j - B0   (FP64; one wavefront per pattern, four per workgroup)
 __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__ K32,
                                                        const bf16_t *__restrict__ K16,
                                                        const bf16_t *__restrict__ K16lo,
@@ -379,11 +399,11 @@ __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__
                                                        const double *__restrict__ scal, bool centred,
                                                        double *__restrict__ cst)
 {
-    __shared__ double red[4];
-    const int i = blockIdx.x;
+    const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= I) return;  // a whole wavefront
     double acc = 0.0;
-    if (centred && K8) {  // byte counts (exact): 16 per thread and 16-byte load (Pp % 16 == 0)
-        for (int j = threadIdx.x * 16; j < Pp; j += 256 * 16) {
+    if (centred && K8) {  // byte counts (exact): 16 per lane and 16-byte load (Pp % 16 == 0)
+        for (int j = lane * 16; j < Pp; j += 64 * 16) {
             const uint4 v = *reinterpret_cast<const uint4 *>(K8 + (size_t)i * Pp + j);
             const uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
@@ -396,7 +416,7 @@ __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__
             }
         }
     } else if (centred) {
-        for (int j = threadIdx.x; j < Pp; j += 256) {
+        for (int j = lane; j < Pp; j += 64) {
             float k;
             if (K32) k = K32[(size_t)i * Pp + j];
             else {
@@ -407,10 +427,7 @@ __global__ __launch_bounds__(256) void k_pattern_const(const float *__restrict__
         }
     }
     acc = wave_sum(acc);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        cst[i] = red[0] + red[1] + red[2] + red[3] - scal[SC_B0];
+    if (lane == 0) cst[i] = acc - scal[SC_B0];
 }
 
 }  // namespace
@@ -474,7 +491,8 @@ hipError_t launch_slice_stats(const emc_ctx *c, bool want_colsum, hipStream_t st
 
 hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_bias, dim3(1), dim3(1024), 0, st, c->sumW, c->ns, c->logw, c->S, c->Sp, c->I,
+    const int nb = std::max(1, (std::max(c->Sp, c->Pp) + 1023) / 1024);
+    hipLaunchKernelGGL(k_bias, dim3(nb), dim3(1024), 0, st, c->sumW, c->ns, c->logw, c->S, c->Sp, c->I,
                        c->sumK_total, c->opt.fluence, c->opt.diff, sigma, c->mfma ? c->colsum : nullptr,
                        c->Pa, c->Pp, c->bias, c->rscale, c->colmean, c->scal);
     return hipGetLastError();
@@ -482,7 +500,7 @@ hipError_t launch_bias(const emc_ctx *c, float sigma, hipStream_t st)
 
 hipError_t launch_pattern_const(const emc_ctx *c, bool centred, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pattern_const, dim3(c->I), dim3(256), 0, st, c->K32, c->K16, c->k_split ? c->K16lo : nullptr, c->K8, c->I,
+    hipLaunchKernelGGL(k_pattern_const, dim3((c->I + 3) / 4), dim3(256), 0, st, c->K32, c->K16, c->k_split ? c->K16lo : nullptr, c->K8, c->I,
                        c->Pp, c->colmean, c->sumK, c->scal, centred, c->cst);
     return hipGetLastError();
 }
