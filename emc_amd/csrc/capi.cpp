@@ -224,7 +224,8 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (opt.diff < 0 || opt.diff > 3 || opt.interp < 0 || opt.interp > 1 || opt.precision < 0 || opt.precision > 2 ||
         opt.insert < 0 || opt.insert > 2 || (opt.count_bytes != 0 && opt.count_bytes != 2) || opt.mfma_tile < 0 ||
         opt.mstep < 0 || opt.mstep > 2 || opt.scaling < 0 || opt.scaling > 1 ||
-        opt.mfma_tile > 1)
+        opt.mfma_tile > 1 || opt.tile_rows < 0 || (opt.mfma_staging != 0 && opt.mfma_staging != 1) ||
+        (opt.keep_logits != 0 && opt.keep_logits != 1))
         return fail(nullptr, EMC_E_INVALID, "emc_create: bad option value");
     if (opt.scaling && opt.diff == EMC_DIFF_TRUE_POISSON)
         return fail(nullptr, EMC_E_UNSUPPORTED, "emc_create: intensity scaling needs a chi^2 diff");

@@ -108,3 +108,12 @@ def test_rotation_distribution_is_uniform():
     # (2/pi) sqrt(1 - x^2) on [-1, 1], so the mean of its absolute value is 4/(3 pi)
     assert np.allclose((q.astype(np.float64) ** 2).mean(0), 0.25, atol=0.01)
     assert abs(np.abs(q[:, 0]).mean() - 4 / (3 * np.pi)) < 0.01
+
+
+@pytest.mark.parametrize("bad", [dict(mfma_staging=2), dict(keep_logits=-1), dict(keep_logits=2),
+                                 dict(tile_rows=-1), dict(count_bytes=1), dict(mstep=3)])
+def test_create_rejects_bad_option_values(bad):
+    """emc_create checks every option before it touches a device (so this runs without a GPU)."""
+    with pytest.raises(emc_amd.EmcError) as ei:
+        emc_amd.Engine(32, 60, 10, **bad)
+    assert "bad option value" in str(ei.value)
