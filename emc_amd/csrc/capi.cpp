@@ -846,8 +846,10 @@ static emc_status mstep_enqueue(emc_ctx *c)
 {
     hipStream_t st = c->stream;
     ev_start(c, EMC_STAGE_MSTEP);
-    HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
-    HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
+    if (c->opt.insert != EMC_INSERT_BINNED) {  // the binned insertion's first chunk writes the sums
+        HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
+        HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
+    }
     const bool fused = c->mfma && c->opt.insert == EMC_INSERT_FUSED;
     // MFMA M-step form: both kernels are enqueued in AUTO mode and the one not chosen by the
     // E-step's non-zero count (scal[SC_NNZ]) returns at once, so the host never waits for it
@@ -950,8 +952,10 @@ static emc_status sparse_mstep_enqueue(emc_ctx *c)
 {
     hipStream_t st = c->stream;
     ev_start(c, EMC_STAGE_MSTEP);
-    HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
-    HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
+    if (c->opt.insert != EMC_INSERT_BINNED) {  // the binned insertion writes the sums
+        HIP_TRY(c, launch_fill_f32(c->Mnum, 0.0f, c->N3, st));
+        HIP_TRY(c, launch_fill_f32(c->Wt, 0.0f, c->N3, st));
+    }
     gev_rec(c, 1, 0, 0);
     HIP_TRY(c, launch_grouped_mstep(c, st));
     gev_rec(c, 1, 0, 1);

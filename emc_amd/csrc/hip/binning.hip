@@ -390,7 +390,8 @@ __global__ __launch_bounds__(kThreads) void k_bin_group(const int32_t *__restric
 // fixed order.  gbeg: [key - key0] first group of each brick of the chunk (kBricks + 1 entries).
 __global__ __launch_bounds__(kThreads) void k_insert_reduce(const float *__restrict__ gpart,
                                                             const int32_t *__restrict__ gbeg, BinGeom g,
-                                                            float *__restrict__ Mnum, float *__restrict__ Wt)
+                                                            float *__restrict__ Mnum, float *__restrict__ Wt,
+                                                            bool first)
 {
     const int64_t v = blockIdx.x * (int64_t)kThreads + threadIdx.x;
     const int L = g.L;
@@ -410,8 +411,13 @@ __global__ __launch_bounds__(kThreads) void k_insert_reduce(const float *__restr
                     sw += gpart[(size_t)gi * 2 * kHalo3 + kHalo3 + hv];
                 }
             }
-    Mnum[v] += sm;
-    Wt[v] += sw;
+    if (first) {  // the pass's first rotation chunk: the sums start here (no separate zero fill)
+        Mnum[v] = sm;
+        Wt[v] = sw;
+    } else {
+        Mnum[v] += sm;
+        Wt[v] += sw;
+    }
 }
 
 BinGeom geom_of(const emc_ctx *c)
@@ -609,7 +615,12 @@ hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st
     const int k = s0 / c->chunk;
     const int b0 = c->bin_chunk_blocks[k], b1 = c->bin_chunk_blocks[k + 1];
     const int g0 = c->bin_chunk_groups[k], g1 = c->bin_chunk_groups[k + 1];
-    if (b1 <= b0) return hipSuccess;
+    if (b1 <= b0) {  // no pairs in this chunk; the first chunk still starts the sums at zero
+        hipError_t e = hipSuccess;
+        if (k == 0) e = launch_fill_f32(c->Mnum, 0.0f, c->N3, st);
+        if (e == hipSuccess && k == 0) e = launch_fill_f32(c->Wt, 0.0f, c->N3, st);
+        return e;
+    }
     const BinGeom g = geom_of(c);
     const int32_t *blocks = c->bin_blocks + 4 * (size_t)b0;
     if (c->opt.interp == EMC_INTERP_NEAREST)
@@ -623,7 +634,7 @@ hipError_t launch_insert_binned(const emc_ctx *c, int s0, int cs, hipStream_t st
                        c->bin_gpart);
     const int nbr = g.nb * g.nb * g.nb;
     hipLaunchKernelGGL(k_insert_reduce, dim3((unsigned)((c->N3 + kThreads - 1) / kThreads)), dim3(kThreads), 0, st,
-                       c->bin_gpart, c->bin_gbeg + (size_t)k * (nbr + 1), g, c->Mnum, c->Wt);
+                       c->bin_gpart, c->bin_gbeg + (size_t)k * (nbr + 1), g, c->Mnum, c->Wt, k == 0);
     return hipGetLastError();
 }
 
