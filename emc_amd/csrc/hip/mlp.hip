@@ -10,9 +10,13 @@
 // (v_mfma_f32_32x32x16_bf16) with FP32 accumulation; FP32 master weights and Adam moments.
 //
 // GEMM kernel: C[m, n] = sum_k A(m, k) B(n, k), 64 x 64 tile, 256 threads = 4 wavefronts of one
-// 32 x 32 MFMA tile each, BK = 32.  Each operand is either k-contiguous ("row": X[m][k]) or
-// m-contiguous ("col": X[k][m], a transposed read, used where the reduction runs over the batch
-// or over a weight's output index); col operands are transposed while staged into LDS.
+// 32 x 32 MFMA tile each, BK = 128 (64 where K is not a multiple of 128).  Each operand is either
+// k-contiguous ("row": X[m][k]) or m-contiguous ("col": X[k][m], a transposed read, used where the
+// reduction runs over the batch or over a weight's output index); col operands are transposed
+// while staged into LDS.  The network's products are small (a minibatch of 1,024 rows, K of 448 to
+// 1,024: 56-128 workgroups), so a launch is bound by the latency of its global loads: the next
+// k-step's tile is loaded into registers while the current one is multiplied, and the wide k-step
+// keeps the number of such round trips at K / BK.
 #include <algorithm>
 #include <cmath>
 
@@ -26,7 +30,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 64, GK = 32, GLD = GK + 8;
+constexpr int GT = 64, GK = 32;  // tile; K must be a multiple of GK (BK is chosen per launch)
 constexpr int kD = 1024;
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
@@ -44,42 +48,71 @@ struct MlpEpi {
     int m_valid, n_valid;
 };
 
-// A(m,k) at A[m*lda + k] (row) or A[k*lda + m] (col); likewise B(n,k).
-template <bool ACOL, bool BCOL, int EPI>
+// A(m,k) at A[m*lda + k] (row) or A[k*lda + m] (col); likewise B(n,k).  K % BK == 0.
+template <bool ACOL, bool BCOL, int EPI, int BK>
 __global__ __launch_bounds__(256) void k_mlp_gemm(const bf16_t *__restrict__ A, int lda, const bf16_t *__restrict__ B,
                                                   int ldb, int K, MlpEpi e)
 {
-    __shared__ __attribute__((aligned(16))) bf16_t As[GT][GLD], Bs[GT][GLD];
+    static_assert(BK % 32 == 0, "a 64 x BK tile is BK / 32 chunks of 8 elements per thread");
+    constexpr int LD = BK + 8;  // padded row: 16-byte fragment reads of 32 consecutive rows hit distinct banks
+    constexpr int NC = BK / 32;  // chunks of 8 elements per thread and operand
+    __shared__ __attribute__((aligned(16))) bf16_t As[GT][LD], Bs[GT][LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    // chunk q of this thread: row operand -> row id / (BK / 8), k offset (id % (BK / 8)) * 8 with
+    // id = q * 256 + tid; col operand -> k row NC (tid / 8) + q, 8 consecutive m (or n) from
+    // (tid % 8) * 8, so that the thread holds NC consecutive k of each of its 8 m
+    auto load = [&](const bf16_t *X, int ld, int x0, bool col, int k0, u32x4 *r) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const int id = q * 256 + tid;
+            if (!col)
+                r[q] = *reinterpret_cast<const u32x4 *>(X + (size_t)(x0 + id / (BK / 8)) * ld + k0 + (id % (BK / 8)) * 8);
+            else
+                r[q] = *reinterpret_cast<const u32x4 *>(X + (size_t)(k0 + NC * (tid >> 3) + q) * ld + x0 + (tid & 7) * 8);
+        }
+    };
+    auto stage = [&](bf16_t (*Xs)[LD], bool col, const u32x4 *r) {
+        if (!col) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const int id = q * 256 + tid;
+                *reinterpret_cast<u32x4 *>(&Xs[id / (BK / 8)][(id % (BK / 8)) * 8]) = r[q];
+            }
+        } else {  // transposed into LDS: the NC k of one m as one 2 NC-byte write
+            const int kq = NC * (tid >> 3), mm = (tid & 7) * 8;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                uint32_t h[NC];  // element j of each chunk (k = kq + q)
+#pragma unroll
+                for (int q = 0; q < NC; ++q) h[q] = (j & 1) ? (r[q][j >> 1] >> 16) : (r[q][j >> 1] & 0xffffu);
+                if constexpr (NC == 4)
+                    *reinterpret_cast<uint2 *>(&Xs[mm + j][kq]) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+                else if constexpr (NC == 2)
+                    *reinterpret_cast<uint32_t *>(&Xs[mm + j][kq]) = h[0] | (h[1] << 16);
+                else
+                    Xs[mm + j][kq] = (bf16_t)h[0];
+            }
+        }
+    };
     f32x16 acc;
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        // a 64 x 32 tile is 256 chunks of 8 elements: one per thread and operand
-        if (!ACOL) {
-            const int r = tid >> 2, c = (tid & 3) * 8;
-            *reinterpret_cast<u32x4 *>(&As[r][c]) = *reinterpret_cast<const u32x4 *>(A + (size_t)(m0 + r) * lda + k0 + c);
-        } else {  // 8 consecutive m at one k, transposed into LDS
-            const int kk = tid >> 3, mm = (tid & 7) * 8;
-            const u32x4 v = *reinterpret_cast<const u32x4 *>(A + (size_t)(k0 + kk) * lda + m0 + mm);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) As[mm + q][kk] = (bf16_t)(v[q >> 1] >> ((q & 1) * 16));
-        }
-        if (!BCOL) {
-            const int r = tid >> 2, c = (tid & 3) * 8;
-            *reinterpret_cast<u32x4 *>(&Bs[r][c]) = *reinterpret_cast<const u32x4 *>(B + (size_t)(n0 + r) * ldb + k0 + c);
-        } else {
-            const int kk = tid >> 3, nn = (tid & 7) * 8;
-            const u32x4 v = *reinterpret_cast<const u32x4 *>(B + (size_t)(k0 + kk) * ldb + n0 + nn);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) Bs[nn + q][kk] = (bf16_t)(v[q >> 1] >> ((q & 1) * 16));
-        }
+    u32x4 ra[NC], rb[NC];
+    load(A, lda, m0, ACOL, 0, ra);
+    load(B, ldb, n0, BCOL, 0, rb);
+    const int fr = lane & 31, fk = (lane >> 5) * 8;
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        stage(As, ACOL, ra);
+        stage(Bs, BCOL, rb);
         __syncthreads();
-        const int fr = lane & 31, fk = (lane >> 5) * 8;
+        if (k0 + BK < K) {  // the next k-step's loads are in flight during this step's products
+            load(A, lda, m0, ACOL, k0 + BK, ra);
+            load(B, ldb, n0, BCOL, k0 + BK, rb);
+        }
 #pragma unroll
-        for (int kk = 0; kk < GK; kk += 16) {
+        for (int kk = 0; kk < BK; kk += 16) {
             const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wm + fr][kk + fk]);
             const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[wn + fr][kk + fk]);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
@@ -323,16 +356,18 @@ __global__ void k_union_mass(const float *__restrict__ p, int Cp, int I, const i
 
 // Cell sums of the full responsibilities: Tt[c][i] = sum over rotations s of cell c of
 // (Ph + Pl)[s][i]; with LOGITS also the cell's largest logit Mx[c][i] = max_s Lg[s][i] (-inf for
-// an empty cell).  Grid (pattern blocks, cells); rows are read coalesced.
+// an empty cell).  Grid (pattern blocks, cells); rows are read coalesced.  pdirty (null: read
+// every row) marks the 256-pattern row segments of P that may hold a non-zero; the others hold
+// zeros and are not read (adding their zeros would leave the sums unchanged).
 template <bool LOGITS>
 __global__ __launch_bounds__(256) void k_cell_targets(const bf16_t *__restrict__ Ph, const bf16_t *__restrict__ Pl,
                                                       const float *__restrict__ Lg, int64_t ld, int I,
                                                       const int32_t *__restrict__ cell_rot,
-                                                      const int32_t *__restrict__ rbeg, float *__restrict__ Tt,
-                                                      float *__restrict__ Mx)
+                                                      const int32_t *__restrict__ rbeg, const uint8_t *__restrict__ pdirty,
+                                                      int nseg, float *__restrict__ Tt, float *__restrict__ Mx)
 {
-    // eight patterns per thread: 16-byte loads of hi and lo, two of the logits (i0 < I implies
-    // i0 + 8 <= ld); each pattern's sum runs over the cell's rotations in order
+    // eight patterns per thread (one segment): 16-byte loads of hi and lo, two of the logits
+    // (i0 < I implies i0 + 8 <= ld); each pattern's sum runs over the cell's rotations in order
     const int i0 = (blockIdx.x * 256 + threadIdx.x) * 8, c = blockIdx.y;
     if (i0 >= I) return;
     float acc[8], mx[8];
@@ -342,11 +377,14 @@ __global__ __launch_bounds__(256) void k_cell_targets(const bf16_t *__restrict__
         mx[k] = -INFINITY;
     }
     for (int g = rbeg[c]; g < rbeg[c + 1]; ++g) {
-        const size_t o = (size_t)cell_rot[g] * ld + i0;
-        const u32x4 h = *reinterpret_cast<const u32x4 *>(Ph + o), l = *reinterpret_cast<const u32x4 *>(Pl + o);
+        const int s = cell_rot[g];
+        const size_t o = (size_t)s * ld + i0;
+        if (!pdirty || pdirty[(size_t)s * nseg + (i0 >> 8)]) {
+            const u32x4 h = *reinterpret_cast<const u32x4 *>(Ph + o), l = *reinterpret_cast<const u32x4 *>(Pl + o);
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            acc[k] += bf2f((bf16_t)(h[k >> 1] >> (16 * (k & 1)))) + bf2f((bf16_t)(l[k >> 1] >> (16 * (k & 1))));
+            for (int k = 0; k < 8; ++k)
+                acc[k] += bf2f((bf16_t)(h[k >> 1] >> (16 * (k & 1)))) + bf2f((bf16_t)(l[k >> 1] >> (16 * (k & 1))));
+        }
         if (LOGITS) {
             const float4 a = *reinterpret_cast<const float4 *>(Lg + o), b = *reinterpret_cast<const float4 *>(Lg + o + 4);
             const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -508,7 +546,12 @@ hipError_t mlp_gemm(bool acol, bool bcol, int epi, const bf16_t *A, int lda, con
     const dim3 grid(N / GT, M / GT), blk(256);
 #define EMC_MLP_CASE(AC, BC, EP)                                                                                   \
     if (acol == AC && bcol == BC && epi == EP) {                                                                   \
-        hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP>), grid, blk, 0, st, A, lda, B, ldb, K, e);                      \
+        if (K % 128 == 0)                                                                                          \
+            hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 128>), grid, blk, 0, st, A, lda, B, ldb, K, e);             \
+        else if (K % 64 == 0)                                                                                      \
+            hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 64>), grid, blk, 0, st, A, lda, B, ldb, K, e);              \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 32>), grid, blk, 0, st, A, lda, B, ldb, K, e);              \
         return hipGetLastError();                                                                                  \
     }
     EMC_MLP_CASE(false, false, MLP_EPI_BIAS_GELU)
@@ -601,10 +644,10 @@ hipError_t mlp_cell_targets(const emc_ctx *c, float *Tt, float *Mx, hipStream_t 
     const dim3 grid((c->I + 2047) / 2048, pr->C);  // eight patterns per thread
     if (Mx)
         hipLaunchKernelGGL(k_cell_targets<true>, grid, dim3(256), 0, st, c->Ph, c->Pl, c->R, (int64_t)c->Ip, c->I,
-                           pr->cell_rot, pr->rbeg, Tt, Mx);
+                           pr->cell_rot, pr->rbeg, c->pdirty, p_segments(c), Tt, Mx);
     else
         hipLaunchKernelGGL(k_cell_targets<false>, grid, dim3(256), 0, st, c->Ph, c->Pl, nullptr, (int64_t)c->Ip, c->I,
-                           pr->cell_rot, pr->rbeg, Tt, nullptr);
+                           pr->cell_rot, pr->rbeg, c->pdirty, p_segments(c), Tt, nullptr);
     return hipGetLastError();
 }
 

@@ -66,12 +66,15 @@ def test_mlp_forward_matches_oracle(orc, d):
     assert err < 5e-3  # hidden activations are rounded to BF16 on the device
 
 
-def test_mlp_gradients_match_oracle(orc, d):
-    log("[mlp gradients] H=128 C=60 B=50")
+@pytest.mark.parametrize("rows,batch", [(50, 64), (200, 256)])
+def test_mlp_gradients_match_oracle(orc, d, rows, batch):
+    """The weight gradients reduce over the minibatch: a batch of 64 runs their products with a
+    64-wide k-step, one of 256 with the 128-wide one."""
+    log(f"[mlp gradients] H=128 C=60 B={rows} of a {batch}-row batch")
     rng = np.random.default_rng(6)
     with engine(d, I=256) as e:
-        e.enable_proposal(d["cq"], hidden=128, batch=64)
-        x, t, params = net_case(e, rng)
+        e.enable_proposal(d["cq"], hidden=128, batch=batch)
+        x, t, params = net_case(e, rng, B=rows)
         loss, g = e.mlp_gradients(x, t)
     xb = bf16(x).astype(np.float64)
     acts = orc.mlp_forward(xb, params)
@@ -206,6 +209,31 @@ def test_network_adds_capture_to_recall_at_lower_cost(d):
     assert ct.size == 4 and cf.size == 4
     assert ct.sum() > cf.sum() + 0.05
     assert wt < wf
+
+
+def test_cell_targets_equal_the_cell_sums_of_the_responsibilities(d):
+    """The network's targets Tt[i][c] = sum of P[s][i] over the fine rotations s of cell c, after
+    full searches from the true model: the posteriors are sharp, so most 256-pattern row segments
+    of P hold only zeros and the kernel does not read them."""
+    log("[cell targets] 60 cells, 1,000 patterns, two full searches from the true model")
+    cell = emc_amd.coarse_partition(d["q"], d["cq"])
+    C = d["cq"].shape[0]
+    with engine(d) as e:
+        e.enable_proposal(d["cq"], top_k=8, warmup_full=10, full_every=2, epochs=1, hidden=64, batch=64)
+        for it in range(2):
+            assert e.iterate(it)["search"] == "full"
+        t = e.proposal_cell_targets()
+        P = e.get_resp().astype(np.float64)
+    S, I = P.shape
+    segs = np.pad(P, ((0, 0), (0, -I % 256))).reshape(S, -1, 256)
+    zero = float((segs == 0).all(2).mean())
+    want = np.zeros((I, C))
+    for c in range(C):
+        want[:, c] = P[cell == c].sum(0)
+    err = float(np.abs(t - want).max())
+    log(f"  zero row segments of P {zero:.3f}, max |Tt - cell sums of P| {err:.2e}")
+    assert zero > 0.3
+    assert np.allclose(t, want, rtol=1e-5, atol=1e-7)
 
 
 def check_top_logit_cells(e, d, rec, R):
