@@ -331,6 +331,9 @@ extern "C" emc_status emc_create(const emc_dims *d, const emc_options *o, emc_ct
     if (st == EMC_OK) A(dalloc(&c->rscale, Sp), "rscale");
     if (st == EMC_OK) A(dalloc(&c->cst, Ip), "cst");
     if (st == EMC_OK) A(dalloc(&c->R, Sp * Ip), "R");
+    // the responsibilities pass reads the padding columns [I, Ip) of a row beside the real ones and
+    // gives them p = exp(x - inf) = 0, which needs finite x: nothing else writes them
+    if (st == EMC_OK) A(hipMemset(c->R, 0, sizeof(float) * Sp * Ip), "R");
     if (st == EMC_OK) A(dalloc(&c->lse, Ip), "lse");
     c->part_len = std::max<int64_t>(emc::softmax_part_len(c->S, c->I, softmax_fused_bands(c)), 4096);
     if (st == EMC_OK) A(dalloc(&c->part, c->part_len), "part");

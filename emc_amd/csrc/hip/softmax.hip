@@ -148,7 +148,9 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
     static_assert(kApplyRows == 8, "8 rows: two per wave in the row sums, 8 floats per block read");
     constexpr float kLog2e = 1.4426950408889634f;
     int nz = 0;  // OUT = 1: entries with a non-zero hi or lo term (for the choice of the M-step)
-    __shared__ double rs[kApplyRows][256];  // per-thread row sums
+    // per-thread row sums: each is a float (a float sum of the thread's four values, or one value),
+    // kept as such and widened when the row sums are formed
+    __shared__ float rs[kApplyRows][256];
     const int c0 = blockIdx.x * kApplyCols + threadIdx.x * 4;
     const int r0 = blockIdx.y * kApplyRows;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -225,9 +227,9 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                 if (act[k]) alist[slot++] = (int16_t)(4 * threadIdx.x + k);
             }
             __syncthreads();
-            double ts[kApplyRows];
+            float ts[kApplyRows];
 #pragma unroll
-            for (int rr = 0; rr < kApplyRows; ++rr) ts[rr] = 0.0;
+            for (int rr = 0; rr < kApplyRows; ++rr) ts[rr] = 0.0f;
             if (threadIdx.x < total) {
                 const int lp = alist[threadIdx.x], n = blockIdx.x * kApplyCols + lp;
                 const double l = lse[n];
@@ -246,7 +248,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
 #pragma unroll
                 for (int rr = 0; rr < kApplyRows; ++rr) {
                     const float p = r0 + rr < S ? __builtin_amdgcn_exp2f(((x[rr] - lhn) - lln) * kLog2e) : 0.0f;
-                    ts[rr] = (double)p;
+                    ts[rr] = p;
                     // the dense form's rounding: hi = bf16(p), lo = bf16(p - hi), one conversion each
                     const uint32_t h = pk_bf16(p, 0.0f) & 0xffffu;
                     const uint32_t lo = pk_bf16(p - __builtin_bit_cast(float, h << 16), 0.0f) & 0xffffu;
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
 #pragma unroll
         for (int rr = 0; rr < kApplyRows; ++rr) {
             const int s = r0 + rr;
-            double acc = 0.0;
+            float acc = 0.0f;
             uint2 hv = make_uint2(0u, 0u), lv = make_uint2(0u, 0u);
             int any_nz = 0;
             if (s < S && c0 < I) {
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
                 float p[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) p[k] = __builtin_amdgcn_exp2f(((x[k] - lh[k]) - ll[k]) * kLog2e);
-                acc = (double)((p[0] + p[1]) + (p[2] + p[3]));
+                acc = (p[0] + p[1]) + (p[2] + p[3]);
                 if (OUT == 0) {
                     *reinterpret_cast<float4 *>(src) = make_float4(p[0], p[1], p[2], p[3]);
                 } else {
@@ -357,7 +359,7 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         const int row = 2 * wave + (lane >> 5), part = lane & 31;
         double t = 0.0;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) t += rs[row][part + 32 * q];
+        for (int q = 0; q < 8; ++q) t += (double)rs[row][part + 32 * q];
 #pragma unroll
         for (int o = 16; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
         if (part == 0 && r0 + row < S) Tpart[(size_t)blockIdx.x * S + r0 + row] = t;
