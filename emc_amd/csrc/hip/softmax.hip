@@ -115,10 +115,15 @@ constexpr int kApplyRows = 8;    // rotation rows per workgroup (the block reads
 // Two floats to a packed BF16 pair (a in the low half), round to nearest even: gfx950's
 // two-source v_cvt_pk_bf16_f32, which rounds as f2bf_dev does.  Written as asm because the compiler
 // emits one single-source conversion per value and merges the halves with extra instructions.
+// The s_nop is the wait state gfx950 requires between a transcendental VALU instruction (the
+// v_exp_f32 that produces p) and a VALU instruction that reads its result, and between a write with
+// a destination select and its reader: the compiler inserts these for the instructions it emits but
+// not in front of inline asm, and without it the conversion can read a stale value when scheduled
+// right after the exp.
 __device__ __forceinline__ uint32_t pk_bf16(float a, float b)
 {
     uint32_t r;
-    asm("v_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    asm("s_nop 0\n\tv_cvt_pk_bf16_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
     return r;
 }
 constexpr int kApplyCols = 1024; // pattern columns per workgroup (4 per thread)
