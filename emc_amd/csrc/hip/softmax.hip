@@ -164,11 +164,23 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
     // lse as a float pair: x - lh is exact wherever p is not negligible (x and lse within a factor
     // of 2, Sterbenz), so exp((x - lh) - ll) carries the double-precision argument to float rounding
     // (a column past I gets lh = +inf: its p is exp2(-inf) = 0 without a select per element)
+    // The thread's four patterns' lse, band maxima and block records are read with one vector load
+    // each (lse always: c0 is a multiple of 4 and lse holds Ip >= I rounded up to 4 entries; the
+    // [band][I] rows of pm and blk when I is a multiple of 4), element by element otherwise: the
+    // short workgroups of this kernel are bound by the vector-memory data path.
+    double lv[4];
+    if (c0 < I) {
+        const double2 a = *reinterpret_cast<const double2 *>(lse + c0), b = *reinterpret_cast<const double2 *>(lse + c0 + 2);
+        lv[0] = a.x; lv[1] = a.y; lv[2] = b.x; lv[3] = b.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lv[k] = 0.0;
+    }
     float lh[4], ll[4];
     bool neg[4];  // this band's logits of pattern c0 + k are negligible (not read)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const double l = (c0 + k < I) ? lse[c0 + k] : 0.0;
+        const double l = (c0 + k < I) ? lv[k] : 0.0;
         lh[k] = (c0 + k < I) ? (float)l : INFINITY;
         ll[k] = (c0 + k < I) ? (float)(l - (double)lh[k]) : 0.0f;
         neg[k] = false;
@@ -182,20 +194,43 @@ __global__ __launch_bounds__(256) void k_sm_apply(float *__restrict__ R, int64_t
         one_band = b0 == (rl / chunk) * bpc + (rl % chunk) / 256;
         if (one_band) {  // the workgroup's rows lie in one band (always, when blk is given)
             brow0 = (r0 / chunk) * chunk + ((r0 % chunk) / 256) * 256;
+            float pv[4];
+            int cv[4] = {-2, -2, -2, -2};
+            const size_t o = (size_t)b0 * I + c0;
+            if ((I & 3) == 0 && c0 < I) {  // uniform layout test; then c0 + 3 < I
+                const float4 v = *reinterpret_cast<const float4 *>(pm + o);
+                pv[0] = v.x; pv[1] = v.y; pv[2] = v.z; pv[3] = v.w;
+                if (blk) {
+                    const uint2 w = *reinterpret_cast<const uint2 *>(blk + o);
+                    cv[0] = (int16_t)(w.x & 0xffffu); cv[1] = (int16_t)(w.x >> 16);
+                    cv[2] = (int16_t)(w.y & 0xffffu); cv[3] = (int16_t)(w.y >> 16);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    pv[k] = c0 + k < I ? pm[o + k] : 0.0f;
+                    if (blk && c0 + k < I) cv[k] = blk[o + k];
+                }
+            }
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (c0 + k < I) {
-                    neg[k] = (double)pm[(size_t)b0 * I + c0 + k] < lse[c0 + k] - kSkipApplyNats;
-                    if (blk) code[k] = blk[(size_t)b0 * I + c0 + k];
+                    neg[k] = (double)pv[k] < lv[k] - kSkipApplyNats;
+                    code[k] = cv[k];
                     neg[k] = neg[k] || code[k] == -1;
                 }
         }
     }
-    // segment flags (a segment past the last reads the last; rows past S read row S - 1: not used)
+    // segment flags (a segment past the last reads the last; rows past S read row S - 1: not used):
+    // the same 8 bytes for the whole wavefront, so lane rr loads row rr's and the others take it by
+    // readlane
     int flag[kApplyRows];
     const int segc = OUT == 1 ? min(seg, nseg - 1) : 0;
+    {
+        const int fv = OUT == 1 && lane < kApplyRows ? (int)pdirty[(size_t)min(r0 + lane, S - 1) * nseg + segc] : 0;
 #pragma unroll
-    for (int rr = 0; rr < kApplyRows; ++rr) flag[rr] = OUT == 1 ? (int)pdirty[(size_t)min(r0 + rr, S - 1) * nseg + segc] : 0;
+        for (int rr = 0; rr < kApplyRows; ++rr) flag[rr] = __builtin_amdgcn_readlane(fv, rr);
+    }
 
     // Sparse form: when at most kSparseMax of the workgroup's patterns have a band that is not
     // negligible, one thread per such pattern computes its 8 values (the others are exactly 0), the
