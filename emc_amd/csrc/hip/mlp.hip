@@ -30,7 +30,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 64, GK = 32;  // tile; K must be a multiple of GK (BK is chosen per launch)
+constexpr int GT = 64, GK = 64;  // tile; K must be a multiple of GK (D, H, Cp and the batch are)
 constexpr int kD = 1024;
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
@@ -53,7 +53,7 @@ template <bool ACOL, bool BCOL, int EPI, int BK>
 __global__ __launch_bounds__(256) void k_mlp_gemm(const bf16_t *__restrict__ A, int lda, const bf16_t *__restrict__ B,
                                                   int ldb, int K, MlpEpi e)
 {
-    static_assert(BK % 32 == 0, "a 64 x BK tile is BK / 32 chunks of 8 elements per thread");
+    static_assert(BK == 64 || BK == 128, "a 64 x BK tile is BK / 32 chunks of 8 elements per thread");
     constexpr int LD = BK + 8;  // padded row: 16-byte fragment reads of 32 consecutive rows hit distinct banks
     constexpr int NC = BK / 32;  // chunks of 8 elements per thread and operand
     __shared__ __attribute__((aligned(16))) bf16_t As[GT][LD], Bs[GT][LD];
@@ -89,10 +89,8 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(const bf16_t *__restrict__ A, 
                 for (int q = 0; q < NC; ++q) h[q] = (j & 1) ? (r[q][j >> 1] >> 16) : (r[q][j >> 1] & 0xffffu);
                 if constexpr (NC == 4)
                     *reinterpret_cast<uint2 *>(&Xs[mm + j][kq]) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
-                else if constexpr (NC == 2)
-                    *reinterpret_cast<uint32_t *>(&Xs[mm + j][kq]) = h[0] | (h[1] << 16);
                 else
-                    Xs[mm + j][kq] = (bf16_t)h[0];
+                    *reinterpret_cast<uint32_t *>(&Xs[mm + j][kq]) = h[0] | (h[1] << 16);
             }
         }
     };
@@ -548,10 +546,8 @@ hipError_t mlp_gemm(bool acol, bool bcol, int epi, const bf16_t *A, int lda, con
     if (acol == AC && bcol == BC && epi == EP) {                                                                   \
         if (K % 128 == 0)                                                                                          \
             hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 128>), grid, blk, 0, st, A, lda, B, ldb, K, e);             \
-        else if (K % 64 == 0)                                                                                      \
-            hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 64>), grid, blk, 0, st, A, lda, B, ldb, K, e);              \
         else                                                                                                       \
-            hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 32>), grid, blk, 0, st, A, lda, B, ldb, K, e);              \
+            hipLaunchKernelGGL((k_mlp_gemm<AC, BC, EP, 64>), grid, blk, 0, st, A, lda, B, ldb, K, e);              \
         return hipGetLastError();                                                                                  \
     }
     EMC_MLP_CASE(false, false, MLP_EPI_BIAS_GELU)
