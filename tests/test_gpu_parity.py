@@ -653,8 +653,8 @@ def test_repeated_runs_are_bit_identical(small, prec):
         assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("chunk,skips", [(512, True), (1000, True), (700, False)])
-def test_skipped_logit_stores_leave_the_iteration_unchanged(small, chunk, skips):
+@pytest.mark.parametrize("chunk,skips,I", [(512, True, None), (1000, True, None), (700, False, None), (512, True, 333)])
+def test_skipped_logit_stores_leave_the_iteration_unchanged(small, chunk, skips, I):
     """emc_iterate leaves out the stores of a 256-rotation band's logits for a pattern whose band
     maximum lies 128 nats below a maximum already found for it (keep_logits = 0, the default) and
     packs the kept ones of a tile into whole-line blocks; the softmax reads the blocks and takes the
@@ -663,12 +663,13 @@ def test_skipped_logit_stores_leave_the_iteration_unchanged(small, chunk, skips)
     chunks of a size that is not a multiple of 8 store every logit.  Over four iterations (the
     posteriors sharpen), models, weights, log-sum-exps, responsibilities, slice sums and total
     log-likelihoods equal those of keep_logits = 1 bit for bit, and the incomplete logits are not
-    handed out."""
-    log(f"[logit store skipping] chunk={chunk}")
+    handed out.  A ragged pattern count (333) takes the responsibilities pass's element loads of the
+    per-pattern state instead of its vector loads."""
+    log(f"[logit store skipping] chunk={chunk} patterns={I or 'all'}")
     d = small
     outs, skipped = [], []
     for keep in (1, 0):
-        with engine(d, "bf16x2", insert="binned", blur_sigma=0.5, slice_chunk=chunk, keep_logits=keep) as e:
+        with engine(d, "bf16x2", I=I, insert="binned", blur_sigma=0.5, slice_chunk=chunk, keep_logits=keep) as e:
             sts = [e.iterate(it) for it in range(4)]
             M, Wt = e.get_model()
             outs.append(([s["total_logl"] for s in sts], M, Wt, e.get_lse(), e.get_resp(), e.get_slice_weights()))
